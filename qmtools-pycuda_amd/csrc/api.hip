@@ -1,0 +1,244 @@
+// Context, workspaces and molecule tables of the qmgrid C-ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "qm_internal.h"
+
+#define QM_VERSION_MAJOR 1
+#define QM_VERSION_MINOR 0
+#define QM_VERSION_PATCH 0
+
+int qm_fail(qm_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return code;
+}
+
+int qm_enter(qm_ctx* ctx) {
+  if (!ctx) return QM_EINVAL;
+  ctx->err.clear();
+  QM_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+  return QM_OK;
+}
+
+int qm_ws(qm_ctx* ctx, int slot, size_t bytes, void** out) {
+  Workspace& w = ctx->ws[slot];
+  if (w.bytes < bytes) {
+    if (w.ptr) {
+      // the old buffer may still be read by queued work
+      QM_CHECK_HIP(ctx, hipDeviceSynchronize());
+      QM_CHECK_HIP(ctx, hipFree(w.ptr));
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)4096);
+    want = (want + 4095) & ~(size_t)4095;
+    if (hipMalloc(&w.ptr, want) != hipSuccess) {
+      (void)hipGetLastError();
+      w.ptr = nullptr;
+      return qm_fail(ctx, QM_ENOMEM, "workspace %d: cannot allocate %zu bytes", slot, want);
+    }
+    w.bytes = want;
+  }
+  *out = w.ptr;
+  return QM_OK;
+}
+
+int qm_check_geom(qm_ctx* ctx, const qm_geom* g) {
+  QM_REQUIRE(ctx, g != nullptr, QM_EINVAL, "null geometry");
+  QM_REQUIRE(ctx, g->nx > 0 && g->ny > 0 && g->nz > 0, QM_EINVAL, "grid sizes must be positive");
+  QM_REQUIRE(ctx, (int64_t)g->nx * g->ny * g->nz < ((int64_t)1 << 34), QM_ELIMIT, "grid too large");
+  QM_REQUIRE(ctx, std::isfinite(g->step) && g->step > 0, QM_EINVAL, "grid step must be positive");
+  return QM_OK;
+}
+
+extern "C" {
+
+int qm_version(void) { return QM_VERSION_MAJOR * 10000 + QM_VERSION_MINOR * 100 + QM_VERSION_PATCH; }
+
+int qm_ctx_create(int device, qm_ctx** out) {
+  if (!out) return QM_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return QM_EHIP;
+  }
+  if (device < 0 || device >= n) return QM_EINVAL;
+  qm_ctx* c = new (std::nothrow) qm_ctx();
+  if (!c) return QM_ENOMEM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipHostMalloc((void**)&c->host_scalars, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return QM_EHIP;
+  }
+  *out = c;
+  return QM_OK;
+}
+
+int qm_ctx_destroy(qm_ctx* ctx) {
+  if (!ctx) return QM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  for (auto& w : ctx->ws)
+    if (w.ptr) (void)hipFree(w.ptr);
+  if (ctx->host_scalars) (void)hipHostFree(ctx->host_scalars);
+  qm_fft_cache_free(ctx->fft);
+  delete ctx;
+  return QM_OK;
+}
+
+const char* qm_last_error(const qm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static void free_mol_device(qm_mol* m) {
+  if (m->atoms) (void)hipFree(m->atoms);
+  if (m->shells) (void)hipFree(m->shells);
+  if (m->prims) (void)hipFree(m->prims);
+  if (m->P64) (void)hipFree(m->P64);
+  if (m->P16) (void)hipFree(m->P16);
+  if (m->Pb16) (void)hipFree(m->Pb16);
+  qm_gto_cache_free(m->gto);
+  m->gto = nullptr;
+  m->atoms = nullptr; m->shells = nullptr; m->prims = nullptr;
+  m->P64 = nullptr; m->P16 = nullptr; m->Pb16 = nullptr;
+}
+
+int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr, int nshell,
+                  const int* shell_atom, const int* shell_L, const int* shell_nprim,
+                  const int* shell_poff, int nprim, const double* alpha, const double* coeff,
+                  qm_mol** out) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, out != nullptr, QM_EINVAL, "null output handle");
+  *out = nullptr;
+  QM_REQUIRE(ctx, natoms > 0 && nshell > 0 && nprim > 0, QM_EINVAL, "empty molecule");
+  QM_REQUIRE(ctx, Z && xyz_bohr && shell_atom && shell_L && shell_nprim && shell_poff && alpha && coeff,
+             QM_EINVAL, "null table pointer");
+  qm_mol* m = new (std::nothrow) qm_mol();
+  QM_REQUIRE(ctx, m != nullptr, QM_ENOMEM, "host allocation failed");
+  m->natoms = natoms;
+  m->nshell = nshell;
+  m->nprim = nprim;
+  m->xyz.assign(xyz_bohr, xyz_bohr + 3 * natoms);
+  m->Z.resize(natoms);
+  double qtot = 0;
+  for (int a = 0; a < natoms; ++a) {
+    if (Z[a] <= 0 || !std::isfinite(xyz_bohr[3 * a]) || !std::isfinite(xyz_bohr[3 * a + 1]) ||
+        !std::isfinite(xyz_bohr[3 * a + 2])) {
+      delete m;
+      return qm_fail(ctx, QM_EINVAL, "atom %d: bad charge or coordinates", a);
+    }
+    m->Z[a] = Z[a];
+    qtot += Z[a];
+  }
+  m->qtot = qtot;
+  int bf = 0;
+  m->h_shells.resize(nshell);
+  for (int s = 0; s < nshell; ++s) {
+    DevShell d{};
+    d.atom = shell_atom[s];
+    d.L = shell_L[s];
+    d.nprim = shell_nprim[s];
+    d.poff = shell_poff[s];
+    d.bf = bf;
+    if (d.atom < 0 || d.atom >= natoms || d.L < 0 || d.L > 2 || d.nprim <= 0 || d.poff < 0 ||
+        d.poff + d.nprim > nprim) {
+      delete m;
+      return qm_fail(ctx, QM_EINVAL, "shell %d: bad table entry", s);
+    }
+    bf += 2 * d.L + 1;
+    m->h_shells[s] = d;
+  }
+  m->nbf = bf;
+  m->nbf_pad = (bf + 63) / 64 * 64;
+  m->h_alpha.assign(alpha, alpha + nprim);
+  m->h_coeff.assign(coeff, coeff + nprim);
+
+  std::vector<double4> at(natoms);
+  for (int a = 0; a < natoms; ++a)
+    at[a] = make_double4(xyz_bohr[3 * a], xyz_bohr[3 * a + 1], xyz_bohr[3 * a + 2], (double)Z[a]);
+  std::vector<double2> pr(nprim);
+  for (int p = 0; p < nprim; ++p) pr[p] = make_double2(alpha[p], coeff[p]);
+
+  if (hipMalloc(&m->atoms, sizeof(double4) * natoms) != hipSuccess ||
+      hipMalloc(&m->shells, sizeof(DevShell) * nshell) != hipSuccess ||
+      hipMalloc(&m->prims, sizeof(double2) * nprim) != hipSuccess) {
+    (void)hipGetLastError();
+    free_mol_device(m);
+    delete m;
+    return qm_fail(ctx, QM_ENOMEM, "molecule tables: device allocation failed");
+  }
+  if (hipMemcpy(m->atoms, at.data(), sizeof(double4) * natoms, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->shells, m->h_shells.data(), sizeof(DevShell) * nshell, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->prims, pr.data(), sizeof(double2) * nprim, hipMemcpyHostToDevice) != hipSuccess) {
+    hipError_t e = hipGetLastError();
+    free_mol_device(m);
+    delete m;
+    return qm_fail(ctx, QM_EHIP, "molecule tables: upload failed: %s", hipGetErrorString(e));
+  }
+  *out = m;
+  return QM_OK;
+}
+
+int qm_mol_set_dm(qm_ctx* ctx, qm_mol* mol, const double* dm) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, mol && dm, QM_EINVAL, "null argument");
+  const int n = mol->nbf;
+  mol->h_P.assign((size_t)n * n, 0.0);
+  for (int p = 0; p < n; ++p) {
+    for (int q = 0; q <= p; ++q) {
+      double v = dm[(size_t)p * n + q];
+      QM_REQUIRE(ctx, std::isfinite(v), QM_EINVAL, "density matrix element (%d,%d) is not finite", p, q);
+      mol->h_P[(size_t)p * n + q] = 2.0 * v;
+      mol->h_P[(size_t)q * n + p] = 2.0 * v;
+    }
+  }
+  rc = qm_density_upload_P(ctx, mol);
+  if (rc) return rc;
+  mol->has_dm = true;
+  mol->gto_dirty = true;
+  return QM_OK;
+}
+
+int qm_mol_set_coords(qm_ctx* ctx, qm_mol* mol, const double* xyz_bohr) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, mol && xyz_bohr, QM_EINVAL, "null argument");
+  std::vector<double4> at(mol->natoms);
+  for (int a = 0; a < mol->natoms; ++a) {
+    QM_REQUIRE(ctx, std::isfinite(xyz_bohr[3 * a]) && std::isfinite(xyz_bohr[3 * a + 1]) &&
+                        std::isfinite(xyz_bohr[3 * a + 2]),
+               QM_EINVAL, "atom %d: coordinates are not finite", a);
+    at[a] = make_double4(xyz_bohr[3 * a], xyz_bohr[3 * a + 1], xyz_bohr[3 * a + 2], mol->Z[a]);
+  }
+  mol->xyz.assign(xyz_bohr, xyz_bohr + 3 * mol->natoms);
+  // queued kernels may still read the old table
+  QM_CHECK_HIP(ctx, hipDeviceSynchronize());
+  QM_CHECK_HIP(ctx, hipMemcpy(mol->atoms, at.data(), sizeof(double4) * mol->natoms, hipMemcpyHostToDevice));
+  mol->gto_dirty = true;
+  return QM_OK;
+}
+
+int qm_mol_nbf(const qm_mol* mol) { return mol ? mol->nbf : -1; }
+
+int qm_mol_destroy(qm_ctx* ctx, qm_mol* mol) {
+  if (!mol) return QM_OK;
+  if (ctx) (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  free_mol_device(mol);
+  delete mol;
+  return QM_OK;
+}
+
+}  // extern "C"

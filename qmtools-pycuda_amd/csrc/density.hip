@@ -1,0 +1,609 @@
+// Electron density on the grid: basis-on-grid + MFMA contraction.
+//
+// The reference (kernel_density.cu:34-155) recomputes phi_p and phi_q for
+// every (p, q) pair at every voxel.  Here the work is split in two kernels
+// per chunk of points:
+//
+//   1. k_phi_*:      Phi[t][f] for all points t of the chunk and basis
+//                    functions f (one thread per point, shells are wave
+//                    uniform, writes coalesced along t);
+//   2. k_contract_*: rho_t = sum_n Phi[t][n] (Phi P)[t][n], the diagonal of
+//                    Phi P Phi^T, as an MFMA GEMM with the row-dot fused
+//                    into the epilogue (no X = Phi P round trip to HBM).
+//
+// P is the total density matrix 2*(tril(dm) + tril(dm,-1)^T): the reference
+// reads only the lower triangle of dm and doubles off-diagonal terms
+// (kernel_density.cu:74-80,138) and multiplies by 2 for the closed shell (:147).
+//
+// Layouts:
+//   FP64:   Phi^T[f * ld + t]            (f64 MFMA 16x16x4: A lane l = (t: l%16, k: l/16))
+//   F16/BF16: K8-interleaved [(f/8) * ld + t] * 8 + f%8, hi and lo planes
+//           (MFMA 16x16x32: A lane l = (t: l%16, k: 8*(l/16)+0..7), one 16-byte load)
+//   P uses the same layouts with nbf_pad in place of ld (P is symmetric).
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "qm_internal.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+constexpr int kPhiThreads = 256;
+constexpr int kTileM = 128;      // points per contraction workgroup (4 waves x 32 rows)
+constexpr int kTileN = 64;       // columns per n-block
+constexpr int kThreads = 256;
+
+// Real solid harmonic prefactors of the reference (kernel_density.cu:17-28).
+constexpr double C00 = 0.28209479177387814;
+constexpr double C1 = 0.4886025119029199;
+constexpr double C2A = 1.0925484305920792;
+constexpr double C20 = 0.31539156525252005;
+constexpr double C22 = 0.5462742152960396;
+
+// Values of the 2L+1 functions of a shell, m order (0, +1, -1, +2, -2).
+template <typename R>
+__device__ __forceinline__ int shell_values(int L, R x, R y, R z, R radial, R* v) {
+  if (L == 0) {
+    v[0] = (R)C00 * radial;
+    return 1;
+  }
+  if (L == 1) {
+    v[0] = (R)C1 * z * radial;
+    v[1] = (R)C1 * x * radial;
+    v[2] = (R)C1 * y * radial;
+    return 3;
+  }
+  v[0] = (R)C20 * ((R)2 * z * z - x * x - y * y) * radial;
+  v[1] = (R)C2A * x * z * radial;
+  v[2] = (R)C2A * z * y * radial;
+  v[3] = (R)C22 * (x * x - y * y) * radial;
+  v[4] = (R)C2A * x * y * radial;
+  return 5;
+}
+
+// Position of chunk point t: sub-point s = t / nvox of voxel v0 + t % nvox (cell centred).
+__device__ __forceinline__ void chunk_point(const GeomView& g, int64_t v0, int nvox, int nsub, int t, double& x,
+                                            double& y, double& z) {
+  const int s = t / nvox;
+  const int64_t v = v0 + (t - (int64_t)s * nvox);
+  const int64_t i = v % g.nx;
+  const int64_t j = (v / g.nx) % g.ny;
+  const int64_t k = v / ((int64_t)g.nx * g.ny);
+  const int sx = s % nsub, sy = (s / nsub) % nsub, sz = s / (nsub * nsub);
+  const double inv = 1.0 / nsub;
+  x = g.ox + ((double)i + (sx + 0.5) * inv) * g.h;
+  y = g.oy + ((double)j + (sy + 0.5) * inv) * g.h;
+  z = g.oz + ((double)k + (sz + 0.5) * inv) * g.h;
+}
+
+// exp(-a r2) == 0 in double for a r2 > 745.2: a shell whose smallest exponent
+// passes that bound contributes exact zeros, so skipping it changes nothing.
+__global__ __launch_bounds__(kPhiThreads) void k_phi_f64(MolView m, GeomView g, int64_t v0, int nvox, int nsub,
+                                                         int npts, const double* __restrict__ pts, int ld,
+                                                         int nbf_pad, double* __restrict__ phi) {
+  const int t = blockIdx.x * kPhiThreads + threadIdx.x;
+  if (t >= ld) return;
+  if (t >= npts) {
+    for (int f = 0; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.0;
+    return;
+  }
+  double x, y, z;
+  if (pts) {
+    x = pts[3 * (int64_t)t];
+    y = pts[3 * (int64_t)t + 1];
+    z = pts[3 * (int64_t)t + 2];
+  } else {
+    chunk_point(g, v0, nvox, nsub, t, x, y, z);
+  }
+  for (int s = 0; s < m.nshell; ++s) {
+    const DevShell sh = m.shells[s];
+    const double4 a = m.atoms[sh.atom];
+    const double dx = x - a.x, dy = y - a.y, dz = z - a.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double radial = 0.0;
+    for (int p = 0; p < sh.nprim; ++p) {
+      const double2 pc = m.prims[sh.poff + p];
+      radial += pc.y * exp(-pc.x * r2);
+    }
+    double v[5];
+    const int nc = shell_values<double>(sh.L, dx, dy, dz, radial, v);
+    for (int c = 0; c < nc; ++c) phi[(int64_t)(sh.bf + c) * ld + t] = v[c];
+  }
+  for (int f = m.nbf; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.0;
+}
+
+template <typename T16>
+__device__ __forceinline__ T16 to16(float x);
+template <>
+__device__ __forceinline__ _Float16 to16<_Float16>(float x) { return (_Float16)x; }
+template <>
+__device__ __forceinline__ __bf16 to16<__bf16>(float x) { return (__bf16)x; }
+
+// FP32 basis values, split into 16-bit hi + lo after scaling each point by a
+// power of two that brings its largest |phi| to [1, 2): exact scaling, and it
+// keeps small-valued points (far from all nuclei) out of the FP16 subnormals.
+// Pass 1 finds the largest |phi|, pass 2 recomputes and writes.
+template <typename T16>
+__global__ __launch_bounds__(kPhiThreads) void k_phi_split(MolView m, GeomView g, int64_t v0, int nvox, int nsub,
+                                                           int npts, int ld, int nbf_pad, T16* __restrict__ hi,
+                                                           T16* __restrict__ lo, float* __restrict__ pscale) {
+  __shared__ T16 stage_hi[kPhiThreads][8];
+  __shared__ T16 stage_lo[kPhiThreads][8];
+  const int t = blockIdx.x * kPhiThreads + threadIdx.x;
+  if (t >= ld) return;
+  const bool live = t < npts;
+  float x = 0.f, y = 0.f, z = 0.f;
+  if (live) {
+    double xd, yd, zd;
+    chunk_point(g, v0, nvox, nsub, t, xd, yd, zd);
+    x = (float)xd;
+    y = (float)yd;
+    z = (float)zd;
+  }
+  float vmax = 0.f;
+  if (live) {
+    for (int s = 0; s < m.nshell; ++s) {
+      const DevShell sh = m.shells[s];
+      const double4 a = m.atoms[sh.atom];
+      const float dx = x - (float)a.x, dy = y - (float)a.y, dz = z - (float)a.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      float radial = 0.f;
+      for (int p = 0; p < sh.nprim; ++p) {
+        const double2 pc = m.prims[sh.poff + p];
+        radial += (float)pc.y * __expf(-(float)pc.x * r2);
+      }
+      float v[5];
+      const int nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
+      for (int c = 0; c < nc; ++c) vmax = fmaxf(vmax, fabsf(v[c]));
+    }
+  }
+  int e = 0;
+  if (vmax > 0.f) frexpf(vmax, &e);          // vmax = f 2^e, f in [0.5, 1)
+  const int sh_e = 1 - e;                    // scaled max in [1, 2)
+  if (pscale) pscale[t] = live ? ldexpf(1.0f, -2 * sh_e) : 0.f;
+  for (int s = 0, f = 0; s < m.nshell; ++s) {
+    const DevShell sh = m.shells[s];
+    float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    int nc = 2 * sh.L + 1;
+    if (live) {
+      const double4 a = m.atoms[sh.atom];
+      const float dx = x - (float)a.x, dy = y - (float)a.y, dz = z - (float)a.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      float radial = 0.f;
+      for (int p = 0; p < sh.nprim; ++p) {
+        const double2 pc = m.prims[sh.poff + p];
+        radial += (float)pc.y * __expf(-(float)pc.x * r2);
+      }
+      nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
+    }
+    for (int c = 0; c < nc; ++c, ++f) {
+      const float sv = ldexpf(v[c], sh_e);
+      const T16 h = to16<T16>(sv);
+      const T16 l = to16<T16>(sv - (float)h);
+      stage_hi[threadIdx.x][f & 7] = h;
+      stage_lo[threadIdx.x][f & 7] = l;
+      if ((f & 7) == 7) {
+        const int64_t o = ((int64_t)(f >> 3) * ld + t) * 8;
+        *(uint4*)(hi + o) = *(const uint4*)stage_hi[threadIdx.x];
+        *(uint4*)(lo + o) = *(const uint4*)stage_lo[threadIdx.x];
+      }
+    }
+    if (s == m.nshell - 1) {
+      // pad the last group and the remaining rows with zeros
+      for (; f < nbf_pad; ++f) {
+        stage_hi[threadIdx.x][f & 7] = to16<T16>(0.f);
+        stage_lo[threadIdx.x][f & 7] = to16<T16>(0.f);
+        if ((f & 7) == 7) {
+          const int64_t o = ((int64_t)(f >> 3) * ld + t) * 8;
+          *(uint4*)(hi + o) = *(const uint4*)stage_hi[threadIdx.x];
+          *(uint4*)(lo + o) = *(const uint4*)stage_lo[threadIdx.x];
+        }
+      }
+    }
+  }
+}
+
+// sum over the 16 lanes that share lane/16 (the n index of the MFMA D layout)
+__device__ __forceinline__ double sum16(double v) {
+  v += __shfl_xor(v, 8, 64);
+  v += __shfl_xor(v, 4, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 1, 64);
+  return v;
+}
+__device__ __forceinline__ float sum16f(float v) {
+  v += __shfl_xor(v, 8, 64);
+  v += __shfl_xor(v, 4, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 1, 64);
+  return v;
+}
+
+template <typename TO>
+__device__ __forceinline__ void store_rho(TO* out, int64_t idx, double v) { out[idx] = (TO)v; }
+
+// rho_t * scale for the 128 points of this workgroup; MFMA f64 16x16x4.
+// out_direct: rows map to out[v0 + t] (subgrid 1); else rows go to rho_tmp[t].
+template <typename TO>
+__global__ __launch_bounds__(kThreads) void k_contract_f64(const double* __restrict__ phi, int ld,
+                                                           const double* __restrict__ P, int nbf_pad, int npts,
+                                                           double scale, TO* __restrict__ out, int64_t v0) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int row0 = blockIdx.x * kTileM + wave * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  double part[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  for (int n0 = 0; n0 < nbf_pad; n0 += kTileN) {
+    d4 acc[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = d4{0, 0, 0, 0};
+    const double* pa = phi + (int64_t)lk * ld + row0 + li;
+    const double* pb = P + (int64_t)lk * nbf_pad + n0 + li;
+    for (int k0 = 0; k0 < nbf_pad; k0 += 4) {
+      const double a0 = pa[(int64_t)k0 * ld];
+      const double a1 = pa[(int64_t)k0 * ld + 16];
+      double b[4];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) b[cb] = pb[(int64_t)k0 * nbf_pad + cb * 16];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        acc[0][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b[cb], acc[0][cb], 0, 0, 0);
+        acc[1][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b[cb], acc[1][cb], 0, 0, 0);
+      }
+    }
+    // epilogue: D[i = 4*lk + r][j = li]  ->  point row0 + rb*16 + i, column n0 + cb*16 + j
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const double* ph = phi + (int64_t)(n0 + cb * 16 + li) * ld + row0 + rb * 16 + 4 * lk;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[rb][r] += acc[rb][cb][r] * ph[r];
+      }
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double v = sum16(part[rb][r]);
+      const int t = row0 + rb * 16 + 4 * lk + r;
+      if (li == 0 && t < npts) store_rho<TO>(out, v0 + t, v * scale);
+    }
+}
+
+template <typename T16>
+struct Vec8;
+template <>
+struct Vec8<_Float16> {
+  typedef h8 type;
+  static __device__ __forceinline__ f4 mma(const h8& a, const h8& b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Vec8<__bf16> {
+  typedef b8 type;
+  static __device__ __forceinline__ f4 mma(const b8& a, const b8& b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+// Split-precision contraction: X = Ahi Bhi + Ahi Blo + Alo Bhi (FP32 accumulate),
+// rho_t = pscale_t * pfac * sum_n (Ahi + Alo)[t][n] X[t][n].
+template <typename T16, typename TO>
+__global__ __launch_bounds__(kThreads) void k_contract_split(const T16* __restrict__ phi_hi,
+                                                             const T16* __restrict__ phi_lo, int ld,
+                                                             const float* __restrict__ pscale,
+                                                             const T16* __restrict__ p_hi,
+                                                             const T16* __restrict__ p_lo, int nbf_pad,
+                                                             int npts, double scale, TO* __restrict__ out,
+                                                             int64_t v0) {
+  typedef typename Vec8<T16>::type V8;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int row0 = blockIdx.x * kTileM + wave * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  float part[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  for (int n0 = 0; n0 < nbf_pad; n0 += kTileN) {
+    f4 acc[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f4{0, 0, 0, 0};
+    for (int k0 = 0; k0 < nbf_pad; k0 += 32) {
+      const int64_t kg = (k0 >> 3) + lk;
+      V8 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int64_t o = (kg * ld + row0 + rb * 16 + li) * 8;
+        ah[rb] = *(const V8*)(phi_hi + o);
+        al[rb] = *(const V8*)(phi_lo + o);
+      }
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const int64_t o = (kg * nbf_pad + n0 + cb * 16 + li) * 8;
+        bh[cb] = *(const V8*)(p_hi + o);
+        bl[cb] = *(const V8*)(p_lo + o);
+      }
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          acc[rb][cb] = Vec8<T16>::mma(al[rb], bh[cb], acc[rb][cb]);
+          acc[rb][cb] = Vec8<T16>::mma(ah[rb], bl[cb], acc[rb][cb]);
+          acc[rb][cb] = Vec8<T16>::mma(ah[rb], bh[cb], acc[rb][cb]);
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const int n = n0 + cb * 16 + li;
+        const int64_t base = ((int64_t)(n >> 3) * ld + row0 + rb * 16 + 4 * lk) * 8 + (n & 7);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float ph = (float)phi_hi[base + 8 * r] + (float)phi_lo[base + 8 * r];
+          part[rb][r] += acc[rb][cb][r] * ph;
+        }
+      }
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float v = sum16f(part[rb][r]);
+      const int t = row0 + rb * 16 + 4 * lk + r;
+      if (li == 0 && t < npts) store_rho<TO>(out, v0 + t, (double)v * (double)pscale[t] * scale);
+    }
+}
+
+// Sum the nsub^3 sub-point densities of each voxel: out[v0 + v] = sum_s rho[s * nvox + v].
+template <typename TO>
+__global__ void k_subgrid_sum(const double* __restrict__ rho, int nvox, int nsub3, TO* __restrict__ out,
+                              int64_t v0) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+  double acc = 0.0;
+  for (int s = 0; s < nsub3; ++s) acc += rho[(int64_t)s * nvox + v];
+  out[v0 + v] = (TO)acc;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------- host side
+namespace {
+
+// P (symmetric) -> 16-bit hi/lo planes in the K8-interleaved layout, scaled by pfac.
+template <typename T16>
+__global__ void k_split_P(const double* __restrict__ P64, int npad, float pfac, T16* __restrict__ hi,
+                          T16* __restrict__ lo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)npad * npad) return;
+  const int k = (int)(idx / npad), n = (int)(idx % npad);
+  const float x = (float)(P64[idx] * (double)pfac);
+  const T16 h = to16<T16>(x);
+  const T16 l = to16<T16>(x - (float)h);
+  const int64_t o = ((int64_t)(k >> 3) * npad + n) * 8 + (k & 7);
+  hi[o] = h;
+  lo[o] = l;
+}
+
+int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+constexpr size_t kPhiBudget = (size_t)4 << 30;   // bytes of basis values per chunk
+
+}  // namespace
+
+int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol) {
+  const int n = mol->nbf, np = mol->nbf_pad;
+  std::vector<double> P((size_t)np * np, 0.0);
+  double pmax = 0.0;
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) {
+      const double v = mol->h_P[(size_t)p * n + q];
+      P[(size_t)p * np + q] = v;
+      pmax = std::max(pmax, std::fabs(v));
+    }
+  int e = 0;
+  if (pmax > 0) std::frexp(pmax, &e);
+  mol->pfac = std::ldexp(1.0, 1 - e);   // largest |P| scaled into [1, 2)
+  const int64_t tot = (int64_t)np * np;
+  if (!mol->P64) {
+    if (hipMalloc(&mol->P64, sizeof(double) * tot) != hipSuccess ||
+        hipMalloc(&mol->P16, sizeof(uint16_t) * 2 * tot) != hipSuccess ||
+        hipMalloc(&mol->Pb16, sizeof(uint16_t) * 2 * tot) != hipSuccess) {
+      (void)hipGetLastError();
+      return qm_fail(ctx, QM_ENOMEM, "density matrix: device allocation failed");
+    }
+  }
+  // queued kernels may still read the previous matrix
+  QM_CHECK_HIP(ctx, hipDeviceSynchronize());
+  QM_CHECK_HIP(ctx, hipMemcpy(mol->P64, P.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  const unsigned blocks = (unsigned)((tot + 255) / 256);
+  hipLaunchKernelGGL(k_split_P<_Float16>, dim3(blocks), dim3(256), 0, 0, (const double*)mol->P64, np,
+                     (float)mol->pfac, (_Float16*)mol->P16, (_Float16*)mol->P16 + tot);
+  QM_CHECK_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_split_P<__bf16>, dim3(blocks), dim3(256), 0, 0, (const double*)mol->P64, np,
+                     (float)mol->pfac, (__bf16*)mol->Pb16, (__bf16*)mol->Pb16 + tot);
+  QM_CHECK_LAUNCH(ctx);
+  QM_CHECK_HIP(ctx, hipDeviceSynchronize());
+  return QM_OK;
+}
+
+namespace {
+
+template <typename TO>
+int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int nsub, TO* out) {
+  const int np = mol->nbf_pad;
+  const int nsub3 = nsub * nsub * nsub;
+  const int64_t nvox_tot = npoints(g);
+  int64_t maxpts = (int64_t)(kPhiBudget / ((size_t)np * sizeof(double)));
+  maxpts = std::max<int64_t>(maxpts / kTileM * kTileM, kTileM);
+  int64_t nvox = std::max<int64_t>(maxpts / nsub3, 1);
+  nvox = std::min<int64_t>(nvox, nvox_tot);
+  nvox = std::min<int64_t>(nvox, ((int64_t)1 << 30) / nsub3);
+  const int64_t ldmax = round_up(nvox * nsub3, kTileM);
+  void* phi = nullptr;
+  int rc = qm_ws(ctx, WS_PHI, sizeof(double) * (size_t)np * ldmax, &phi);
+  if (rc) return rc;
+  double* rho = nullptr;
+  if (nsub3 > 1) {
+    rc = qm_ws(ctx, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
+    if (rc) return rc;
+  }
+  const GeomView gv = geom_view(g);
+  const MolView mv = mol->view();
+  const double h3 = g->step * g->step * g->step;
+  for (int64_t v0 = 0; v0 < nvox_tot; v0 += nvox) {
+    const int nv = (int)std::min<int64_t>(nvox, nvox_tot - v0);
+    const int npts = nv * nsub3;
+    const int ld = (int)round_up(npts, kTileM);
+    hipLaunchKernelGGL(k_phi_f64, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s, mv, gv,
+                       v0, nv, nsub, npts, (const double*)nullptr, ld, np, (double*)phi);
+    QM_CHECK_LAUNCH(ctx);
+    if (nsub3 == 1) {
+      hipLaunchKernelGGL(k_contract_f64<TO>, dim3(ld / kTileM), dim3(kThreads), 0, s, (const double*)phi, ld,
+                         (const double*)mol->P64, np, npts, h3, out, v0);
+    } else {
+      hipLaunchKernelGGL(k_contract_f64<double>, dim3(ld / kTileM), dim3(kThreads), 0, s, (const double*)phi,
+                         ld, (const double*)mol->P64, np, npts, h3 / nsub3, rho, (int64_t)0);
+      QM_CHECK_LAUNCH(ctx);
+      hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
+                         nsub3, out, v0);
+    }
+    QM_CHECK_LAUNCH(ctx);
+  }
+  return QM_OK;
+}
+
+template <typename T16, typename TO>
+int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int nsub, TO* out) {
+  const int np = mol->nbf_pad;
+  const int nsub3 = nsub * nsub * nsub;
+  const int64_t nvox_tot = npoints(g);
+  int64_t maxpts = (int64_t)(kPhiBudget / ((size_t)np * 2 * sizeof(uint16_t)));
+  maxpts = std::max<int64_t>(maxpts / kTileM * kTileM, kTileM);
+  int64_t nvox = std::max<int64_t>(maxpts / nsub3, 1);
+  nvox = std::min<int64_t>(nvox, nvox_tot);
+  nvox = std::min<int64_t>(nvox, ((int64_t)1 << 30) / nsub3);
+  const int64_t ldmax = round_up(nvox * nsub3, kTileM);
+  void* phi = nullptr;
+  int rc = qm_ws(ctx, WS_PHI, sizeof(uint16_t) * 2 * (size_t)np * ldmax, &phi);
+  if (rc) return rc;
+  float* pscale = nullptr;
+  rc = qm_ws(ctx, WS_PHI2, sizeof(float) * ldmax, (void**)&pscale);
+  if (rc) return rc;
+  double* rho = nullptr;
+  if (nsub3 > 1) {
+    rc = qm_ws(ctx, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
+    if (rc) return rc;
+  }
+  const T16* Phi = (const T16*)(std::is_same<T16, _Float16>::value ? mol->P16 : mol->Pb16);
+  const T16* Plo = Phi + (int64_t)np * np;
+  const GeomView gv = geom_view(g);
+  const MolView mv = mol->view();
+  const double h3 = g->step * g->step * g->step / mol->pfac;
+  for (int64_t v0 = 0; v0 < nvox_tot; v0 += nvox) {
+    const int nv = (int)std::min<int64_t>(nvox, nvox_tot - v0);
+    const int npts = nv * nsub3;
+    const int ld = (int)round_up(npts, kTileM);
+    T16* hi = (T16*)phi;
+    T16* lo = hi + (int64_t)np * ld;
+    hipLaunchKernelGGL(k_phi_split<T16>, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s,
+                       mv, gv, v0, nv, nsub, npts, ld, np, hi, lo, pscale);
+    QM_CHECK_LAUNCH(ctx);
+    if (nsub3 == 1) {
+      hipLaunchKernelGGL((k_contract_split<T16, TO>), dim3(ld / kTileM), dim3(kThreads), 0, s, (const T16*)hi,
+                         (const T16*)lo, ld, (const float*)pscale, Phi, Plo, np, npts, h3, out, v0);
+    } else {
+      hipLaunchKernelGGL((k_contract_split<T16, double>), dim3(ld / kTileM), dim3(kThreads), 0, s,
+                         (const T16*)hi, (const T16*)lo, ld, (const float*)pscale, Phi, Plo, np, npts,
+                         h3 / nsub3, rho, (int64_t)0);
+      QM_CHECK_LAUNCH(ctx);
+      hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
+                         nsub3, out, v0);
+    }
+    QM_CHECK_LAUNCH(ctx);
+  }
+  return QM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qm_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int subgrid, int precision,
+               int out_dtype, void* out, int normalize, double* raw_total) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  rc = qm_check_geom(ctx, g);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, mol && out, QM_EINVAL, "density: null argument");
+  QM_REQUIRE(ctx, mol->has_dm, QM_EINVAL, "density: molecule has no density matrix");
+  QM_REQUIRE(ctx, subgrid >= 1 && subgrid <= 16, QM_EINVAL, "density: subgrid must be 1..16");
+  QM_REQUIRE(ctx, out_dtype == QM_F64 || out_dtype == QM_F32, QM_EINVAL, "density: bad output dtype");
+  hipStream_t s = (hipStream_t)stream;
+  const bool f64 = out_dtype == QM_F64;
+  switch (precision) {
+    case QM_PREC_FP64:
+      rc = f64 ? density_fp64<double>(ctx, mol, s, g, subgrid, (double*)out)
+               : density_fp64<float>(ctx, mol, s, g, subgrid, (float*)out);
+      break;
+    case QM_PREC_F16X3:
+      rc = f64 ? density_split<_Float16, double>(ctx, mol, s, g, subgrid, (double*)out)
+               : density_split<_Float16, float>(ctx, mol, s, g, subgrid, (float*)out);
+      break;
+    case QM_PREC_BF16X3:
+      rc = f64 ? density_split<__bf16, double>(ctx, mol, s, g, subgrid, (double*)out)
+               : density_split<__bf16, float>(ctx, mol, s, g, subgrid, (float*)out);
+      break;
+    default:
+      return qm_fail(ctx, QM_EINVAL, "density: bad precision %d", precision);
+  }
+  if (rc) return rc;
+  if (normalize || raw_total) {
+    double* dtot = nullptr;
+    rc = qm_ws(ctx, WS_SCALAR, 256, (void**)&dtot);
+    if (rc) return rc;
+    rc = qm_reduce_dev(ctx, s, QM_RED_SUM, out_dtype, out, nullptr, npoints(g), dtot);
+    if (rc) return rc;
+    if (raw_total) {
+      QM_CHECK_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, dtot, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    if (normalize) {
+      rc = qm_scale_dev(ctx, s, out_dtype, out, npoints(g), nullptr, mol->qtot, dtot);
+      if (rc) return rc;
+    }
+    if (raw_total) {
+      QM_CHECK_HIP(ctx, hipStreamSynchronize(s));
+      *raw_total = ctx->host_scalars[0];
+    }
+  }
+  return QM_OK;
+}
+
+int qm_basis_points(qm_ctx* ctx, qm_mol* mol, void* stream, const double* pts, int64_t npts, double* phi) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, mol && pts && phi, QM_EINVAL, "basis_points: null argument");
+  QM_REQUIRE(ctx, npts > 0 && npts < ((int64_t)1 << 30), QM_ELIMIT, "basis_points: bad point count");
+  // writes phi[f * npts + t] for f < nbf (ld = npts); rows beyond nbf are not stored
+  const int n = (int)npts;
+  const MolView mv = mol->view();
+  GeomView gv{};
+  gv.nx = 1; gv.ny = 1; gv.nz = 1; gv.h = 1.0;
+  hipLaunchKernelGGL(k_phi_f64, dim3((n + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0,
+                     (hipStream_t)stream, mv, gv, (int64_t)0, n, 1, n, pts, n, mol->nbf, phi);
+  QM_CHECK_LAUNCH(ctx);
+  return QM_OK;
+}
+
+}  // extern "C"

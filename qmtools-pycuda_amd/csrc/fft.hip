@@ -1,0 +1,546 @@
+// FFT Poisson solve of the total charge: hand-written mixed-radix Stockham FFT.
+//
+// Reference kernel_fft.cu:23-106 + cuFFT (qmtools.py:2578-2639):
+//   rhs = -( -Q/h^3 + sum_A Z_A (2 pi s^2)^-3/2 exp(-r^2/2s^2) [r^2 <= (cutoff s)^2] ) / eps0
+//   (Angstrom / volt units; voxel centres (i+1/2) h + origin; nuclei at coords * 0.529177211;
+//   Q read x-fastest, rhs written C order [i][j][k])
+//   periodic:  V^ = rhs^ * (-1/(4 pi^2 |k|^2)), V^(0) = 0, k in cycles/Angstrom
+//   free space (no reference counterpart): V = rhs (*) G on the 2x zero-padded grid,
+//     G(d) = -h^3/(4 pi |d|), G(0) = -h^3 C0/(4 pi h), C0 = 2 (3 asinh(1/sqrt2) - pi/4)
+//     (the average of 1/|u| over the voxel), G^ cached per geometry.
+//
+// Layout: complex buffer in C order [n0][n1][n2] (axis 2 contiguous).  One
+// kernel per axis; a workgroup loads B lines into LDS (coalesced: for axis 2
+// the lines are contiguous, otherwise the B lines are adjacent along axis 2),
+// runs the Stockham stages out of place between two LDS planes, and writes
+// back.  Twiddles exp(-2 pi i m/n) come from a per-length table in LDS; a
+// radix-R stage uses the same table for its R-point DFT.  Zero padding is
+// exploited by transforming only the lines that can be non-zero.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "qm_internal.h"
+
+namespace {
+
+constexpr double kRBohr = 0.529177211;       // kernel_fft.cu:15
+constexpr double kEps0 = 0.0055263494;       // kernel_fft.cu:16, e/(V Angstrom)
+constexpr double kAng2Bor = 1.8897259886;
+constexpr int kThreads = 256;
+constexpr int kMaxStages = 24;
+
+template <typename R>
+struct Cx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
+  return Cx<R>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+struct AxisPlan {
+  int n;              // transform length
+  int nstage;
+  int radix[kMaxStages];
+  int span[kMaxStages];
+};
+
+struct AxisLines {
+  int64_t stride;     // distance between consecutive elements of a line
+  int nl0, nl1;       // lines: l0 in [0, nl0), l1 in [0, nl1)
+  int64_t s0, s1;     // line base = l0 * s0 + l1 * s1
+  int B;              // lines per workgroup (consecutive l1)
+};
+
+// R-point DFT of x (forward sign through the table; the inverse uses a conjugated table).
+// w1 = table index step of exp(-2 pi i / R).
+template <typename R, int RR>
+__device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int w1) {
+  if (RR == 2) {
+    const Cx<R> a = x[0], b = x[1];
+    x[0] = Cx<R>{a.x + b.x, a.y + b.y};
+    x[1] = Cx<R>{a.x - b.x, a.y - b.y};
+  } else if (RR == 4) {
+    const Cx<R> a0{x[0].x + x[2].x, x[0].y + x[2].y}, a1{x[0].x - x[2].x, x[0].y - x[2].y};
+    const Cx<R> b0{x[1].x + x[3].x, x[1].y + x[3].y}, b1{x[1].x - x[3].x, x[1].y - x[3].y};
+    const Cx<R> b1w = cmul(b1, tw[w1]);   // exp(-+ i pi/2)
+    x[0] = Cx<R>{a0.x + b0.x, a0.y + b0.y};
+    x[1] = Cx<R>{a1.x + b1w.x, a1.y + b1w.y};
+    x[2] = Cx<R>{a0.x - b0.x, a0.y - b0.y};
+    x[3] = Cx<R>{a1.x - b1w.x, a1.y - b1w.y};
+  } else {
+    Cx<R> y[RR];
+#pragma unroll
+    for (int j = 0; j < RR; ++j) {
+      Cx<R> acc = x[0];
+#pragma unroll
+      for (int q = 1; q < RR; ++q) {
+        const Cx<R> t = cmul(x[q], tw[((j * q) % RR) * w1]);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
+      y[j] = acc;
+    }
+#pragma unroll
+    for (int j = 0; j < RR; ++j) x[j] = y[j];
+  }
+}
+
+// One radix-RR Stockham stage in place: all inputs of the thread's butterflies
+// are read into registers, then (after a barrier) the outputs are written.
+template <typename R, int RR>
+__device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restrict__ tw, int n, int B, int p) {
+  constexpr int MAXB = (16 + RR - 1) / RR;   // B * n <= 4096: at most ceil(16/RR) butterflies per thread
+  const int m = n / RR;
+  const int tstep = n / (p * RR);
+  const int total = B * m;
+  Cx<R> x[MAXB][RR];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int bf = threadIdx.x + b * kThreads;
+    if (bf < total) {
+      const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
+      const Cx<R>* src = buf + l * n;
+#pragma unroll
+      for (int j = 0; j < RR; ++j) {
+        Cx<R> v = src[i + j * m];
+        if (k) v = cmul(v, tw[(j * k * tstep) % n]);
+        x[b][j] = v;
+      }
+      dft<R, RR>(x[b], tw, m);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int bf = threadIdx.x + b * kThreads;
+    if (bf < total) {
+      const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
+      Cx<R>* dst = buf + l * n + (i - k) * RR + k;
+#pragma unroll
+      for (int j = 0; j < RR; ++j) dst[j * p] = x[b][j];
+    }
+  }
+  __syncthreads();
+}
+
+// Any radix, out of place between two LDS planes (used when a length has a prime factor > 7).
+template <typename R>
+__device__ __forceinline__ void stage_generic(const Cx<R>* in, Cx<R>* out, const Cx<R>* __restrict__ tw, int n,
+                                              int B, int RR, int p) {
+  const int m = n / RR;
+  const int tstep = n / (p * RR);
+  for (int bf = threadIdx.x; bf < B * m; bf += kThreads) {
+    const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
+    const Cx<R>* src = in + l * n;
+    Cx<R>* dst = out + l * n + (i - k) * RR + k;
+    for (int j = 0; j < RR; ++j) {
+      Cx<R> acc{0, 0};
+      for (int q = 0; q < RR; ++q) {
+        Cx<R> v = src[i + q * m];
+        if (k) v = cmul(v, tw[(q * k * tstep) % n]);
+        const Cx<R> t = cmul(v, tw[((j * q) % RR) * m]);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
+      dst[j * p] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// One FFT pass along one axis.  INV: inverse (conjugated) transform, unnormalised.
+// inplace: all radices <= 8 (LDS = table + B lines); otherwise two planes of B lines.
+template <typename R, bool INV>
+__global__ __launch_bounds__(kThreads) void k_fft_axis(Cx<R>* __restrict__ data, const Cx<R>* __restrict__ twg,
+                                                       AxisPlan plan, AxisLines ln, int inplace) {
+  extern __shared__ unsigned char smem[];
+  const int n = plan.n;
+  const int B = ln.B;
+  Cx<R>* tw = (Cx<R>*)smem;
+  Cx<R>* buf0 = tw + n;
+  Cx<R>* buf1 = buf0 + (int64_t)B * n;
+  const int lines_per_row = (ln.nl1 + B - 1) / B;
+  const int64_t wg = blockIdx.x;
+  const int64_t l0 = wg / lines_per_row;
+  const int l1b = (int)(wg % lines_per_row) * B;
+  for (int q = threadIdx.x; q < n; q += kThreads) {
+    Cx<R> w = twg[q];
+    if (INV) w.y = -w.y;
+    tw[q] = w;
+  }
+  const int64_t base0 = l0 * ln.s0 + (int64_t)l1b * ln.s1;
+  const int nb = min(B, ln.nl1 - l1b);
+  for (int e = threadIdx.x; e < B * n; e += kThreads) {
+    int l, a;
+    if (ln.stride == 1) {
+      l = e / n;
+      a = e - l * n;
+    } else {
+      l = e % B;
+      a = e / B;
+    }
+    Cx<R> v{0, 0};
+    if (l < nb) v = data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride];
+    buf0[l * n + a] = v;
+  }
+  __syncthreads();
+  Cx<R>* res = buf0;
+  if (inplace) {
+    for (int s = 0; s < plan.nstage; ++s) {
+      const int p = plan.span[s];
+      switch (plan.radix[s]) {
+        case 2: stage_inplace<R, 2>(buf0, tw, n, B, p); break;
+        case 3: stage_inplace<R, 3>(buf0, tw, n, B, p); break;
+        case 4: stage_inplace<R, 4>(buf0, tw, n, B, p); break;
+        case 5: stage_inplace<R, 5>(buf0, tw, n, B, p); break;
+        case 7: stage_inplace<R, 7>(buf0, tw, n, B, p); break;
+        default: stage_inplace<R, 8>(buf0, tw, n, B, p); break;
+      }
+    }
+  } else {
+    Cx<R>* in = buf0;
+    Cx<R>* out = buf1;
+    for (int s = 0; s < plan.nstage; ++s) {
+      stage_generic<R>(in, out, tw, n, B, plan.radix[s], plan.span[s]);
+      Cx<R>* t = in;
+      in = out;
+      out = t;
+    }
+    res = in;
+  }
+  for (int e = threadIdx.x; e < B * n; e += kThreads) {
+    int l, a;
+    if (ln.stride == 1) {
+      l = e / n;
+      a = e - l * n;
+    } else {
+      l = e % B;
+      a = e / B;
+    }
+    if (l < nb) data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride] = res[l * n + a];
+  }
+}
+
+// rhs, C order, into a [m0][m1][m2] complex buffer (m >= n: zero padding already in place).
+// Tile: 32 i x 32 k at fixed j; Q is read along i (x-fastest) and written along k.
+template <typename R, typename TI>
+__global__ __launch_bounds__(kThreads) void k_rhs(const TI* __restrict__ q, const double4* __restrict__ atoms,
+                                                  int natoms, GeomView g, double sigma, double cut2, int m1, int m2,
+                                                  Cx<R>* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int i0 = blockIdx.x * 32, k0 = blockIdx.z * 32, j = blockIdx.y;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, k = k0 + r;
+    double v = 0.0;
+    if (i < g.nx && k < g.nz) v = (double)q[i + (int64_t)g.nx * (j + (int64_t)g.ny * k)];
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  const double step = g.h / kAng2Bor;
+  const double norm = 1.0 / (sigma * sigma * sigma * 2.0 * M_PI * sqrt(2.0 * M_PI));
+  const double inv2s2 = 1.0 / (2.0 * sigma * sigma);
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, k = k0 + tx;
+    if (i >= g.nx || k >= g.nz) continue;
+    double rho = tile[tx][r] / (-(step * step * step));
+    const double x = g.ox / kAng2Bor + (i + 0.5) * step;
+    const double y = g.oy / kAng2Bor + (j + 0.5) * step;
+    const double z = g.oz / kAng2Bor + (k + 0.5) * step;
+    for (int a = 0; a < natoms; ++a) {
+      const double4 A = atoms[a];
+      const double dx = x - A.x * kRBohr, dy = y - A.y * kRBohr, dz = z - A.z * kRBohr;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 <= cut2) rho += A.w * norm * exp(-r2 * inv2s2);
+    }
+    out[((int64_t)i * m1 + j) * m2 + k] = Cx<R>{(R)(-rho / kEps0), (R)0};
+  }
+}
+
+// periodic Poisson in k space, normalisation of the inverse transform folded in (kernel_fft.cu:82-106)
+template <typename R>
+__global__ void k_poisson_periodic(Cx<R>* __restrict__ a, int n0, int n1, int n2, double step, double scale) {
+  const int64_t N = (int64_t)n0 * n1 * n2;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int k = (int)(idx % n2), j = (int)((idx / n2) % n1), i = (int)(idx / ((int64_t)n1 * n2));
+  if (idx == 0) {
+    a[0] = Cx<R>{0, 0};
+    return;
+  }
+  const double kx = (i > n0 / 2 - 1 ? (double)(i - n0) : (double)i) / (n0 * step);
+  const double ky = (j > n1 / 2 - 1 ? (double)(j - n1) : (double)j) / (n1 * step);
+  const double kz = (k > n2 / 2 - 1 ? (double)(k - n2) : (double)k) / (n2 * step);
+  const double f = -1.0 / (4.0 * M_PI * M_PI * (kx * kx + ky * ky + kz * kz)) * scale;
+  Cx<R> v = a[idx];
+  v.x = (R)(v.x * f);
+  v.y = (R)(v.y * f);
+  a[idx] = v;
+}
+
+template <typename R>
+__global__ void k_mul_real(Cx<R>* __restrict__ a, const R* __restrict__ gh, int64_t N) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const R f = gh[idx];
+  Cx<R> v = a[idx];
+  v.x *= f;
+  v.y *= f;
+  a[idx] = v;
+}
+
+// g(d) on the 2n cyclic grid, C order; d_i = i h for i <= n, (i - 2n) h otherwise
+template <typename R>
+__global__ void k_green(Cx<R>* __restrict__ a, int n0, int n1, int n2, double h_ang) {
+  const int m0 = 2 * n0, m1 = 2 * n1, m2 = 2 * n2;
+  const int64_t N = (int64_t)m0 * m1 * m2;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int k = (int)(idx % m2), j = (int)((idx / m2) % m1), i = (int)(idx / ((int64_t)m1 * m2));
+  const double x = (i <= n0 ? i : i - m0) * h_ang;
+  const double y = (j <= n1 ? j : j - m1) * h_ang;
+  const double z = (k <= n2 ? k : k - m2) * h_ang;
+  const double r = sqrt(x * x + y * y + z * z);
+  const double c0 = 2.0 * (3.0 * asinh(1.0 / sqrt(2.0)) - M_PI / 4.0);
+  const double g = (idx == 0) ? c0 / h_ang : 1.0 / r;
+  a[idx] = Cx<R>{(R)g, (R)0};
+}
+
+template <typename R>
+__global__ void k_green_store(const Cx<R>* __restrict__ a, R* __restrict__ gh, int64_t N, double scale) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  gh[idx] = (R)((double)a[idx].x * scale);
+}
+
+template <typename R, typename TO>
+__global__ void k_extract(const Cx<R>* __restrict__ a, int m1, int m2, int n0, int n1, int n2, TO* __restrict__ out) {
+  const int64_t N = (int64_t)n0 * n1 * n2;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int k = (int)(idx % n2), j = (int)((idx / n2) % n1), i = (int)(idx / ((int64_t)n1 * n2));
+  out[idx] = (TO)a[((int64_t)i * m1 + j) * m2 + k].x;
+}
+
+// ------------------------------------------------------------------ host: plans
+std::vector<int> factorize(int n) {
+  std::vector<int> f;
+  for (int r : {8, 4, 2, 3, 5, 7}) {
+    while (n % r == 0) {
+      f.push_back(r);
+      n /= r;
+    }
+  }
+  for (int p = 11; n > 1; p += 2)
+    while (n % p == 0) {
+      f.push_back(p);
+      n /= p;
+    }
+  return f;
+}
+
+}  // namespace
+
+struct FftPlanCache {
+  std::map<std::pair<int, int>, void*> twiddles;            // (n, bytes per real) -> device table
+  struct Green {
+    int n0, n1, n2;
+    double h;
+    int rbytes;
+    void* gh;
+  };
+  std::vector<Green> greens;
+};
+
+void qm_fft_cache_free(FftPlanCache* c) {
+  if (!c) return;
+  for (auto& kv : c->twiddles) (void)hipFree(kv.second);
+  for (auto& g : c->greens) (void)hipFree(g.gh);
+  delete c;
+}
+
+namespace {
+
+template <typename R>
+int twiddle_table(qm_ctx* ctx, int n, const Cx<R>** out) {
+  if (!ctx->fft) ctx->fft = new FftPlanCache();
+  auto key = std::make_pair(n, (int)sizeof(R));
+  auto it = ctx->fft->twiddles.find(key);
+  if (it != ctx->fft->twiddles.end()) {
+    *out = (const Cx<R>*)it->second;
+    return QM_OK;
+  }
+  std::vector<Cx<R>> h(n);
+  for (int m = 0; m < n; ++m) {
+    // exp(-2 pi i m / n) with the angle reduced exactly in integers
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+    h[m] = Cx<R>{(R)std::cos(ang), (R)std::sin(ang)};
+  }
+  void* d = nullptr;
+  QM_CHECK_HIP(ctx, hipMalloc(&d, sizeof(Cx<R>) * n));
+  QM_CHECK_HIP(ctx, hipMemcpy(d, h.data(), sizeof(Cx<R>) * n, hipMemcpyHostToDevice));
+  ctx->fft->twiddles[key] = d;
+  *out = (const Cx<R>*)d;
+  return QM_OK;
+}
+
+// one pass along `axis` of a C-order [m0][m1][m2] buffer, on the lines whose
+// other two indices are below (lim_a, lim_b) (in increasing axis order)
+template <typename R, bool INV>
+int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, int lim_a, int lim_b) {
+  const int n = m[axis];
+  const Cx<R>* tw = nullptr;
+  int rc = twiddle_table<R>(ctx, n, &tw);
+  if (rc) return rc;
+  AxisPlan plan{};
+  plan.n = n;
+  std::vector<int> f = factorize(n);
+  QM_REQUIRE(ctx, (int)f.size() <= kMaxStages, QM_ELIMIT, "fft: too many stages for length %d", n);
+  int span = 1;
+  plan.nstage = (int)f.size();
+  for (size_t q = 0; q < f.size(); ++q) {
+    plan.radix[q] = f[q];
+    plan.span[q] = span;
+    span *= f[q];
+  }
+  const int64_t st[3] = {(int64_t)m[1] * m[2], (int64_t)m[2], 1};
+  AxisLines ln{};
+  ln.stride = st[axis];
+  int oa = axis == 0 ? 1 : 0;            // other axes, oa < ob
+  int ob = axis == 2 ? 1 : 2;
+  ln.nl0 = lim_a;
+  ln.nl1 = lim_b;
+  ln.s0 = st[oa];
+  ln.s1 = st[ob];
+  bool inplace = true;
+  for (int r : f) inplace = inplace && r <= 8;
+  const int planes = inplace ? 1 : 2;
+  // B lines of n points per workgroup: B * n <= 4096 (the register staging bound of stage_inplace)
+  QM_REQUIRE(ctx, n <= 4096, QM_ELIMIT, "fft: length %d exceeds 4096", n);
+  int B = std::max(1, std::min(16, 4096 / n));
+  if (!inplace) B = std::max(1, std::min(B, 2048 / n));
+  B = std::max(1, std::min(B, ln.nl1));
+  ln.B = B;
+  const int lines_per_row = (ln.nl1 + B - 1) / B;
+  const int64_t nwg = (int64_t)ln.nl0 * lines_per_row;
+  const size_t shmem = sizeof(Cx<R>) * (size_t)n * (1 + (size_t)planes * B);
+  QM_REQUIRE(ctx, nwg <= 0x7fffffff, QM_ELIMIT, "fft: too many lines");
+  if (nwg == 0) return QM_OK;
+  static bool attr_set = false;
+  if (!attr_set) {
+    QM_CHECK_HIP(ctx, hipFuncSetAttribute((const void*)k_fft_axis<R, INV>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_fft_axis<R, INV>), dim3((unsigned)nwg), dim3(kThreads), shmem, s, data, tw, plan, ln,
+                     inplace ? 1 : 0);
+  QM_CHECK_LAUNCH(ctx);
+  return QM_OK;
+}
+
+template <typename R>
+int green_hat(qm_ctx* ctx, hipStream_t s, const qm_geom* g, Cx<R>* work, const R** out) {
+  if (!ctx->fft) ctx->fft = new FftPlanCache();
+  for (auto& gr : ctx->fft->greens)
+    if (gr.n0 == g->nx && gr.n1 == g->ny && gr.n2 == g->nz && gr.h == g->step && gr.rbytes == (int)sizeof(R)) {
+      *out = (const R*)gr.gh;
+      return QM_OK;
+    }
+  const int m[3] = {2 * g->nx, 2 * g->ny, 2 * g->nz};
+  const int64_t M = (int64_t)m[0] * m[1] * m[2];
+  const double h_ang = g->step / kAng2Bor;
+  const unsigned blocks = (unsigned)((M + 255) / 256);
+  hipLaunchKernelGGL(k_green<R>, dim3(blocks), dim3(256), 0, s, work, g->nx, g->ny, g->nz, h_ang);
+  QM_CHECK_LAUNCH(ctx);
+  int rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 2, m[0], m[1]))) return rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 1, m[0], m[2]))) return rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 0, m[1], m[2]))) return rc;
+  void* gh = nullptr;
+  QM_CHECK_HIP(ctx, hipMalloc(&gh, sizeof(R) * M));
+  // V = -(h^3 / 4 pi) IFFT(rhs^ g^) / M   (rhs = -rho/eps0, g = 1/r)
+  const double scale = -(h_ang * h_ang * h_ang) / (4.0 * M_PI) / (double)M;
+  hipLaunchKernelGGL(k_green_store<R>, dim3(blocks), dim3(256), 0, s, (const Cx<R>*)work, (R*)gh, M, scale);
+  QM_CHECK_LAUNCH(ctx);
+  ctx->fft->greens.push_back({g->nx, g->ny, g->nz, g->step, (int)sizeof(R), gh});
+  *out = (const R*)gh;
+  return QM_OK;
+}
+
+template <typename R, typename TIO>
+int hartree_fft(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, const TIO* q, double sigma,
+                double cutoff, int mode, TIO* out) {
+  const bool fs = mode == QM_FFT_FREESPACE;
+  const int n[3] = {g->nx, g->ny, g->nz};
+  const int m[3] = {fs ? 2 * n[0] : n[0], fs ? 2 * n[1] : n[1], fs ? 2 * n[2] : n[2]};
+  const int64_t M = (int64_t)m[0] * m[1] * m[2];
+  Cx<R>* work = nullptr;
+  int rc = qm_ws(ctx, WS_FFT_A, sizeof(Cx<R>) * (size_t)M, (void**)&work);
+  if (rc) return rc;
+  const R* gh = nullptr;
+  if (fs) {
+    if ((rc = green_hat<R>(ctx, s, g, work, &gh))) return rc;
+    QM_CHECK_HIP(ctx, hipMemsetAsync(work, 0, sizeof(Cx<R>) * (size_t)M, s));
+  }
+  const GeomView gv = geom_view(g);
+  const double cut2 = (cutoff * sigma) * (cutoff * sigma);
+  hipLaunchKernelGGL((k_rhs<R, TIO>), dim3((n[0] + 31) / 32, n[1], (n[2] + 31) / 32), dim3(kThreads), 0, s, q,
+                     (const double4*)mol->atoms, mol->natoms, gv, sigma, cut2, m[1], m[2], work);
+  QM_CHECK_LAUNCH(ctx);
+  // forward: z on lines (i < n0, j < n1), y on (i < n0, all k), x on all (j, k)
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 2, n[0], n[1]))) return rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 1, n[0], m[2]))) return rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 0, m[1], m[2]))) return rc;
+  const unsigned blocks = (unsigned)((M + 255) / 256);
+  if (fs) {
+    hipLaunchKernelGGL(k_mul_real<R>, dim3(blocks), dim3(256), 0, s, work, gh, M);
+  } else {
+    const double step_ang = g->step / kAng2Bor;
+    hipLaunchKernelGGL(k_poisson_periodic<R>, dim3(blocks), dim3(256), 0, s, work, m[0], m[1], m[2], step_ang,
+                       1.0 / (double)M);
+  }
+  QM_CHECK_LAUNCH(ctx);
+  // inverse: x on all (j, k), y on (i < n0, all k), z on (i < n0, j < n1)
+  if ((rc = fft_pass<R, true>(ctx, s, work, m, 0, m[1], m[2]))) return rc;
+  if ((rc = fft_pass<R, true>(ctx, s, work, m, 1, n[0], m[2]))) return rc;
+  if ((rc = fft_pass<R, true>(ctx, s, work, m, 2, n[0], n[1]))) return rc;
+  const int64_t N = (int64_t)n[0] * n[1] * n[2];
+  hipLaunchKernelGGL((k_extract<R, TIO>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, (const Cx<R>*)work,
+                     m[1], m[2], n[0], n[1], n[2], out);
+  QM_CHECK_LAUNCH(ctx);
+  return QM_OK;
+}
+
+}  // namespace
+
+extern "C" int qm_hartree_fft(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int precision, int dtype,
+                              const void* q, double sigma, double cutoff, int mode, void* out) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  if ((rc = qm_check_geom(ctx, g))) return rc;
+  QM_REQUIRE(ctx, mol && q && out, QM_EINVAL, "fft: null argument");
+  QM_REQUIRE(ctx, std::isfinite(sigma) && sigma > 0 && std::isfinite(cutoff) && cutoff > 0, QM_EINVAL,
+             "fft: sigma and cutoff must be positive");
+  QM_REQUIRE(ctx, mode == QM_FFT_PERIODIC || mode == QM_FFT_FREESPACE, QM_EINVAL, "fft: bad mode");
+  QM_REQUIRE(ctx, g->nx >= 2 && g->ny >= 2 && g->nz >= 2, QM_EINVAL, "fft: grid too small");
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == QM_PREC_FP64) {
+    if (dtype == QM_F64)
+      return hartree_fft<double, double>(ctx, mol, s, g, (const double*)q, sigma, cutoff, mode, (double*)out);
+    if (dtype == QM_F32)
+      return hartree_fft<double, float>(ctx, mol, s, g, (const float*)q, sigma, cutoff, mode, (float*)out);
+  } else if (precision == QM_PREC_FP32) {
+    if (dtype == QM_F64)
+      return hartree_fft<float, double>(ctx, mol, s, g, (const double*)q, sigma, cutoff, mode, (double*)out);
+    if (dtype == QM_F32)
+      return hartree_fft<float, float>(ctx, mol, s, g, (const float*)q, sigma, cutoff, mode, (float*)out);
+  } else {
+    return qm_fail(ctx, QM_EINVAL, "fft: bad precision %d", precision);
+  }
+  return qm_fail(ctx, QM_EINVAL, "fft: bad dtype");
+}

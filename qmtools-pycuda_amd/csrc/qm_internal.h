@@ -1,0 +1,139 @@
+// Internal declarations shared by the qmgrid translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "qmgrid.h"
+
+// ------------------------------------------------------------------ device-side molecule view
+struct DevShell {
+  int atom;    // index into atoms
+  int L;       // 0..2
+  int nprim;   // primitives
+  int poff;    // first primitive
+  int bf;      // first basis function
+  int pad_[3];
+};
+
+struct MolView {
+  const double4* atoms;   // x, y, z (bohr), Z
+  const DevShell* shells;
+  const double2* prims;   // alpha, coeff
+  int natoms, nshell, nbf;
+};
+
+struct GeomView {
+  int nx, ny, nz;
+  double h;
+  double ox, oy, oz;
+};
+
+inline GeomView geom_view(const qm_geom* g) {
+  GeomView v;
+  v.nx = g->nx; v.ny = g->ny; v.nz = g->nz;
+  v.h = g->step;
+  v.ox = g->origin[0]; v.oy = g->origin[1]; v.oz = g->origin[2];
+  return v;
+}
+
+// ------------------------------------------------------------------ host-side objects
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+enum WsSlot {
+  WS_PHI = 0,      // basis values of a chunk of points
+  WS_PHI2,         // second plane (lo parts / scales)
+  WS_RHO,          // per-sub-point density
+  WS_RED,          // reduction partials
+  WS_SCALAR,       // small device scalars
+  WS_FFT_A,        // FFT work buffers
+  WS_FFT_B,
+  WS_JAC_A,        // Jacobi correction buffers
+  WS_JAC_B,
+  WS_JAC_C,
+  WS_GTO,          // Hermite distributions
+  WS_MISC,
+  WS_COUNT
+};
+
+struct FftPlanCache;
+
+struct qm_ctx {
+  int device = 0;
+  std::string err;
+  Workspace ws[WS_COUNT];
+  double* host_scalars = nullptr;   // pinned host buffer for results
+  FftPlanCache* fft = nullptr;
+};
+
+struct qm_mol {
+  int natoms = 0, nshell = 0, nprim = 0, nbf = 0, nbf_pad = 0;
+  double4* atoms = nullptr;
+  DevShell* shells = nullptr;
+  double2* prims = nullptr;
+  // host copies (used to build per-request tables)
+  std::vector<double> xyz, Z;
+  std::vector<DevShell> h_shells;
+  std::vector<double> h_alpha, h_coeff;
+  // density matrix: total P_eff = 2 * (tril(dm) + tril(dm,-1)^T), nbf_pad^2, zero padded
+  bool has_dm = false;
+  std::vector<double> h_P;          // nbf x nbf total P_eff (host)
+  double* P64 = nullptr;            // [nbf_pad][nbf_pad]
+  void* P16 = nullptr;              // fp16 hi/lo, K8-interleaved, 2 planes
+  void* Pb16 = nullptr;             // bf16 hi/lo, K8-interleaved, 2 planes
+  void* gto = nullptr;              // Hermite distributions of the GTO potential (gto.hip)
+  bool gto_dirty = true;            // set when P or the coordinates change
+  double pfac = 1.0;                // power-of-two scale applied to P before the 16-bit split
+  double qtot = 0.0;
+  MolView view() const {
+    MolView v;
+    v.atoms = atoms; v.shells = shells; v.prims = prims;
+    v.natoms = natoms; v.nshell = nshell; v.nbf = nbf;
+    return v;
+  }
+};
+
+// ------------------------------------------------------------------ error helpers
+int qm_fail(qm_ctx* ctx, int code, const char* fmt, ...);
+
+#define QM_CHECK_HIP(ctx, call)                                                   \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return qm_fail((ctx), QM_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define QM_CHECK_LAUNCH(ctx) QM_CHECK_HIP(ctx, hipGetLastError())
+
+#define QM_REQUIRE(ctx, cond, code, ...)                   \
+  do {                                                     \
+    if (!(cond)) return qm_fail((ctx), (code), __VA_ARGS__); \
+  } while (0)
+
+// grow-only device workspace
+int qm_ws(qm_ctx* ctx, int slot, size_t bytes, void** out);
+int qm_enter(qm_ctx* ctx);   // hipSetDevice + clear error
+
+inline int64_t npoints(const qm_geom* g) { return (int64_t)g->nx * g->ny * g->nz; }
+int qm_check_geom(qm_ctx* ctx, const qm_geom* g);
+
+// ------------------------------------------------------------------ cross-unit internals
+// deterministic FP64 reductions (reduce.hip)
+int qm_reduce_dev(qm_ctx* ctx, hipStream_t s, int op, int dtype, const void* a, const void* b,
+                  int64_t n, double* dev_result);
+int qm_scale_dev(qm_ctx* ctx, hipStream_t s, int dtype, void* a, int64_t n, const double* dev_num,
+                 double num_const, const double* dev_den);
+// density-matrix device forms (density.hip)
+int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol);
+void qm_fft_cache_free(FftPlanCache* c);
+void qm_gto_cache_free(void* c);

@@ -1,0 +1,261 @@
+"""GPU parity of every hot-path entry point against the FP64 oracle (small seeded inputs).
+
+Tolerances: FP64 paths <= 1e-10 relative (normwise: max|diff| / max|ref|);
+mixed paths as stated per test.  Inputs are printed before each GPU step so
+that a failure names its case.
+"""
+import math
+import os
+
+import numpy
+import pytest
+from conftest import BASIS, MOL29766
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = numpy.asarray(a, dtype=numpy.float64).ravel()
+    b = numpy.asarray(b, dtype=numpy.float64).ravel()
+    return float(numpy.abs(a - b).max() / numpy.abs(b).max())
+
+
+@pytest.fixture(scope="module")
+def qm():
+    import qmtools
+    return qmtools
+
+
+@pytest.fixture(scope="module")
+def obasis():
+    from oracle.basisfile import read_basis
+    return read_basis(BASIS)
+
+
+def _system(mol, obasis):
+    from oracle.system import make_system
+    return make_system(mol.types, mol.coords, mol.dm, obasis)
+
+
+@pytest.fixture(scope="module")
+def small(qm, obasis):
+    from qmtools import synth
+    basis = qm.BasisSet(BASIS)
+    mol = synth.synthetic_molecule(6, 21, basis)
+    grid = synth.centred_grid(mol, 32, step_bohr=0.35, pad_ang=2.0)
+    return mol, grid, _system(mol, obasis)
+
+
+@pytest.fixture(scope="module")
+def mol29766(qm, obasis):
+    basis = qm.BasisSet(BASIS)
+    mol = qm.Molecule(os.path.join(MOL29766, "GEOM-B3LYP.xyz"), os.path.join(MOL29766, "D-CCSD.npy"), basis)
+    grid = qm.Grid.DensityGrid(mol, 0.4, 3.0)
+    return mol, grid, _system(mol, obasis)
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+# ---------------------------------------------------------------- density
+@pytest.mark.parametrize("subgrid", [1, 2])
+def test_density_fp64_synthetic(qm, small, subgrid):
+    from oracle import density as od
+    mol, grid, sysm = small
+    nx, ny, nz, h, o = grid.geometry()
+    _say("density fp64 synthetic", grid.shape, "subgrid", subgrid, "nbf", mol.nbf)
+    q = qm.QMTools.Compute_density(grid, mol, subgrid=subgrid)
+    want, raw = od.density_grid(sysm, nx, ny, nz, h, o, subgrid=subgrid)
+    assert _rel(q.qube, want) <= 1e-10
+    assert q.raw_total == pytest.approx(raw, rel=1e-10)
+    assert math.fsum(q.qube.ravel()) == pytest.approx(float(mol.qtot), rel=1e-12)
+
+
+def test_density_fp64_mol29766(qm, mol29766):
+    from oracle import density as od
+    mol, grid, sysm = mol29766
+    nx, ny, nz, h, o = grid.geometry()
+    _say("density fp64 29766", grid.shape)
+    q = qm.QMTools.Compute_density(grid, mol)
+    want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
+    assert _rel(q.qube, want) <= 1e-10
+    assert q.raw_total == pytest.approx(raw, rel=1e-10)
+
+
+@pytest.mark.parametrize("prec,tol", [("f16", 2e-6), ("bf16", 1e-4)])
+def test_density_mixed(qm, small, prec, tol):
+    from oracle import density as od
+    mol, grid, sysm = small
+    nx, ny, nz, h, o = grid.geometry()
+    _say("density", prec, grid.shape)
+    q = qm.QMTools.Compute_density(grid, mol, precision=prec)
+    want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
+    err = _rel(q.qube, want)
+    print(f"{prec}: normwise rho error {err:.3e}, raw total rel err {abs(q.raw_total - raw) / raw:.3e}")
+    assert err <= tol
+    assert abs(q.raw_total - raw) / raw <= tol
+
+
+# ---------------------------------------------------------------- reductions / seed / vne
+def test_reductions(qm, small):
+    from oracle import reduce as orr
+    mol, grid, _ = small
+    rng = numpy.random.default_rng(0)
+    a = qm.Grid.emptyAs(grid)
+    b = qm.Grid.emptyAs(grid)
+    a.LoadFromNPY(rng.standard_normal(a.shape))
+    b.LoadFromNPY(rng.uniform(1.0, 2.0, b.shape))
+    _say("reductions", grid.shape)
+    assert qm.QMTools.Compute_qtot(a) == pytest.approx(orr.total(a.qube), rel=1e-13, abs=1e-12)
+    assert qm.QMTools.Compute_qdiff(a, b) == pytest.approx(orr.absdiff(a.qube, b.qube), rel=1e-13)
+    assert qm.QMTools.Compute_qdiff(a, b, rel=True) == pytest.approx(orr.reldiff(a.qube, b.qube), rel=1e-13)
+    before = a.qube.copy()
+    qm.QMTools.Compute_qscale(a, 2.5, copyBack=True)
+    numpy.testing.assert_array_equal(a.qube, 2.5 * before)
+
+
+def test_seed(qm, small):
+    from oracle import seed as osd
+    mol, grid, _ = small
+    nx, ny, nz, h, o = grid.geometry()
+    _say("seed", grid.shape)
+    s = qm.QMTools.Compute_qseed(grid, mol)
+    want = osd.seed_grid(mol.types, mol.coords, nx, ny, nz, h, o)
+    numpy.testing.assert_allclose(s.qube.ravel(), want, rtol=1e-13, atol=1e-14)
+    assert math.fsum(s.qube.ravel()) == pytest.approx(float(mol.qtot), rel=1e-13)
+
+
+def test_vne_classic(qm, small):
+    from oracle import vne as ov
+    mol, grid, _ = small
+    nx, ny, nz, h, o = grid.geometry()
+    _say("vne classic", grid.shape)
+    v = qm.QMTools.Compute_VNe_classic(grid, mol, divisions=2)
+    want = ov.vne_classic(mol.types, mol.coords, nx, ny, nz, h, o, 2)
+    assert _rel(v.qube, want) <= 1e-12
+
+
+# ---------------------------------------------------------------- Jacobi
+@pytest.mark.parametrize("boundary", ["zero", "multipole"])
+def test_jacobi_fp64_fixed_sweeps(qm, small, boundary):
+    from oracle import density as od
+    from oracle import jacobi as oj
+    from oracle import seed as osd
+    mol, grid, sysm = small
+    nx, ny, nz, h, o = grid.geometry()
+    _say("jacobi fp64", boundary, grid.shape)
+    q = qm.QMTools.Compute_density(grid, mol)
+    v = qm.QMTools.Compute_hartree_iter(q, mol, tolerance=None, max_sweeps=300, boundary=boundary)
+    rho = osd.seed_grid(mol.types, mol.coords, nx, ny, nz, h, o) - q.qube.ravel()
+    b = oj.multipole_boundary(rho, nx, ny, nz, h, o) if boundary == "multipole" else None
+    want = oj.jacobi(rho, nx, ny, nz, h, 300, bvals=b)
+    assert v.sweeps == 300
+    assert _rel(v.qube, want) <= 1e-10
+
+
+def test_jacobi_convergence_loop(qm, small):
+    from oracle import jacobi as oj
+    from oracle import seed as osd
+    mol, grid, _ = small
+    nx, ny, nz, h, o = grid.geometry()
+    _say("jacobi convergence loop", grid.shape)
+    q = qm.QMTools.Compute_density(grid, mol)
+    v = qm.QMTools.Compute_hartree_iter(q, mol, tolerance=1e-6)
+    rho = osd.seed_grid(mol.types, mol.coords, nx, ny, nz, h, o) - q.qube.ravel()
+    want, itr, delta = oj.hartree_iter(rho, nx, ny, nz, h, tolerance=1e-6)
+    assert v.sweeps == 2 * (itr - 1) or v.sweeps == 2 * itr
+    assert _rel(v.qube, want) <= 1e-9
+
+
+def test_jacobi_mixed_2000_sweeps(qm, small):
+    from oracle import jacobi as oj
+    from oracle import seed as osd
+    mol, grid, _ = small
+    nx, ny, nz, h, o = grid.geometry()
+    _say("jacobi mixed 2000", grid.shape)
+    q = qm.QMTools.Compute_density(grid, mol)
+    v = qm.QMTools.Compute_hartree_iter(q, mol, tolerance=None, max_sweeps=2000, boundary="multipole",
+                                        precision="mixed")
+    rho = osd.seed_grid(mol.types, mol.coords, nx, ny, nz, h, o) - q.qube.ravel()
+    b = oj.multipole_boundary(rho, nx, ny, nz, h, o)
+    want = oj.jacobi(rho, nx, ny, nz, h, 2000, bvals=b)
+    err = _rel(v.qube, want)
+    print(f"mixed Jacobi normwise error after 2000 sweeps: {err:.3e}")
+    assert err <= 1e-6
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("mode", ["periodic", "freespace"])
+def test_fft_fp64(qm, small, mode):
+    from oracle import fft as off
+    mol, grid, _ = small
+    nx, ny, nz, h, o = grid.geometry()
+    q = qm.QMTools.Compute_density(grid, mol)
+    _say("fft fp64", mode, grid.shape)
+    v = qm.QMTools.Compute_hartree_fft(q, mol, sigma=0.2, cutoff=6, mode=mode)
+    want = off.hartree_fft(q.qube.ravel(), mol.types, mol.coords, nx, ny, nz, h, o, 0.2, 6, mode=mode)
+    assert v.shape == (nx, ny, nz)
+    assert _rel(v, want) <= 1e-10
+
+
+@pytest.mark.parametrize("shape", [(24, 40, 16), (88, 16, 24)])
+def test_fft_mixed_radix_lengths(qm, small, shape):
+    """Lengths with factors 3, 5 (in-place radices) and 11 (generic radix path)."""
+    from oracle import fft as off
+    mol, _, _ = small
+    nx, ny, nz = shape
+    grid = qm.Grid((1, nx, ny, nz), 0.25)
+    from qmtools.grid import make_float3
+    c = mol.coords.mean(axis=0)
+    grid.origin = make_float3(*(c - 0.5 * numpy.array(shape) * float(grid.step)))
+    rng = numpy.random.default_rng(7)
+    grid.LoadFromNPY(rng.uniform(0.0, 0.01, grid.shape))
+    nx, ny, nz, h, o = grid.geometry()
+    for mode in ("periodic", "freespace"):
+        _say("fft radix", shape, mode)
+        v = qm.QMTools.Compute_hartree_fft(grid, mol, mode=mode)
+        want = off.hartree_fft(grid.qube.ravel(), mol.types, mol.coords, nx, ny, nz, h, o, mode=mode)
+        assert _rel(v, want) <= 1e-10, mode
+
+
+def test_fft_fp32(qm, small):
+    from oracle import fft as off
+    mol, grid, _ = small
+    nx, ny, nz, h, o = grid.geometry()
+    q = qm.QMTools.Compute_density(grid, mol)
+    _say("fft fp32 freespace", grid.shape)
+    v = qm.QMTools.Compute_hartree_fft(q, mol, mode="freespace", precision="mixed")
+    want = off.hartree_fft(q.qube.ravel(), mol.types, mol.coords, nx, ny, nz, h, o, mode="freespace")
+    err = _rel(v, want)
+    print(f"FP32 free-space FFT normwise error {err:.3e}")
+    assert err <= 1e-5
+
+
+# ---------------------------------------------------------------- GTO
+def test_gto_points(qm, small):
+    from oracle import gto as og
+    from qmtools import synth
+    mol, grid, sysm = small
+    pts = synth.uniform_points(grid, 48, 5)
+    # include a nucleus position and a point very close to one (T -> 0 and r < 1e-8 paths)
+    pts = numpy.vstack([pts, mol.coords[:1], mol.coords[1:2] + 1e-9])
+    _say("gto points", pts.shape)
+    v = qm.QMTools.Compute_hartree_points(mol, pts)
+    want = og.potential(sysm, pts)
+    assert _rel(v, want) <= 1e-10
+
+
+def test_gto_grid(qm, obasis):
+    from oracle import gto as og
+    from qmtools import synth
+    basis = qm.BasisSet(BASIS)
+    mol = synth.synthetic_molecule(3, 5, basis)
+    grid = synth.centred_grid(mol, 8, step_bohr=0.9, pad_ang=1.0)
+    nx, ny, nz, h, o = grid.geometry()
+    sysm = _system(mol, obasis)
+    _say("gto grid", grid.shape)
+    vg = qm.Grid.emptyAs(grid)
+    qm.QMTools.Compute_hartree(vg, mol)
+    want = og.potential_grid(sysm, nx, ny, nz, h, o)
+    assert _rel(vg.qube, want) <= 1e-10

@@ -1,0 +1,304 @@
+"""Grid hot-path benchmark: density + Hartree on one MI355X.
+
+Workload (one "step"): seeded 60-atom C/N/O/H molecule in cc-pVDZ (606 basis
+functions), 256^3 cell-centred grid with the molecule centred and 3 Angstrom of
+padding (step 0.112 bohr: 0.1 bohr does not fit the molecule in 256 points);
+  1. density  rho = diag(Phi P Phi^T) on all 256^3 points, normalised to sum Z;
+  2. Hartree potential by FFT Poisson with free-space boundaries (2x zero-padded,
+     512^3 complex transforms) of electrons + Gaussian nuclei;
+  3. analytic GTO-pair + nuclear potential at 10^4 uniformly sampled points.
+All inputs (basis tables, P, points, grids) are resident in HBM before timing.
+
+Default path: plain FP64 (FP64 MFMA contraction, complex128 FFT, FP64 GTO).
+`--precision mixed` measures the mixed path (FP32 basis values, FP16 hi/lo MFMA
+contraction, complex64 FFT; GTO stays FP64).  Each run also measures the other
+path once and reports its throughput and its error against the FP64 GPU result,
+which itself is checked against the FP64 NumPy oracle on a sample of voxels.
+
+Prints one JSON line (see the keys at the end of main()).
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "qmtools-pycuda_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+F16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+class Workload:
+    def __init__(self, natoms=60, seed=2, n=256, npoints=10000, precision="fp64"):
+        import torch
+
+        import qmtools
+        from qmtools import _device, _native as nat, synth
+        from qmtools.qmtools import device_molecule
+        self.torch = torch
+        self.nat = nat
+        self.basis = qmtools.BasisSet(os.path.join(PKG, "qmtools", "data", "cc-pvdz.bin"))
+        self.mol = synth.synthetic_molecule(natoms, seed, self.basis)
+        self.grid = synth.centred_grid(self.mol, n, step_bohr=0.1, pad_ang=3.0)
+        self.n = n
+        self.N = n ** 3
+        self.pts_host = synth.uniform_points(self.grid, npoints, seed + 1)
+        self.npoints = npoints
+        self.precision = precision
+        dev = _device.require()
+        self.dm = device_molecule(self.mol)
+        self.geom = nat.geom(self.grid)
+        self.q64 = torch.empty(self.N, dtype=torch.float64, device=dev)
+        self.v64 = torch.empty(self.N, dtype=torch.float64, device=dev)
+        self.q32 = torch.empty(self.N, dtype=torch.float32, device=dev)
+        self.v32 = torch.empty(self.N, dtype=torch.float32, device=dev)
+        self.pts = torch.from_numpy(self.pts_host.ravel().copy()).to(dev)
+        self.vpts = torch.empty(npoints, dtype=torch.float64, device=dev)
+        self.stream = torch.cuda.current_stream(dev)
+
+    def step(self, precision=None):
+        nat = self.nat
+        p = self.precision if precision is None else precision
+        s = ctypes.c_void_p(self.stream.cuda_stream)
+        g = ctypes.byref(self.geom)
+        if p == "fp64":
+            nat.call("qm_density", self.dm.handle, s, g, 1, nat.QM_PREC_FP64, nat.QM_F64, nat.dptr(self.q64), 1,
+                     None)
+            nat.call("qm_hartree_fft", self.dm.handle, s, g, nat.QM_PREC_FP64, nat.QM_F64, nat.dptr(self.q64),
+                     0.2, 6.0, nat.QM_FFT_FREESPACE, nat.dptr(self.v64))
+        else:
+            nat.call("qm_density", self.dm.handle, s, g, 1, nat.QM_PREC_F16X3, nat.QM_F32, nat.dptr(self.q32), 1,
+                     None)
+            nat.call("qm_hartree_fft", self.dm.handle, s, g, nat.QM_PREC_FP32, nat.QM_F32, nat.dptr(self.q32),
+                     0.2, 6.0, nat.QM_FFT_FREESPACE, nat.dptr(self.v32))
+        nat.call("qm_hartree_gto_points", self.dm.handle, s, nat.dptr(self.pts), self.npoints, nat.dptr(self.vpts))
+
+    def timed(self, steps, warmup, precision=None, profile=False):
+        torch = self.torch
+        nat = self.nat
+        for _ in range(warmup):
+            self.step(precision)
+        torch.cuda.synchronize()
+        ctx = nat.context()
+        if profile:
+            nat.library().qm_profile(ctx, 1)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record(self.stream)
+            self.step(precision)
+            b.record(self.stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if profile:
+            nat.library().qm_profile(ctx, 0)
+        lat = sorted(a.elapsed_time(b) for a, b in ev)
+        return wall, lat
+
+    def kernel_ms(self, kid):
+        tot = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        self.nat.check(self.nat.library().qm_profile_read(self.nat.context(), kid, ctypes.byref(tot),
+                                                          ctypes.byref(n)), self.nat.context())
+        return tot.value, n.value
+
+
+def pct(vals, q):
+    if not vals:
+        return None
+    k = min(len(vals) - 1, max(0, int(math.ceil(q / 100.0 * len(vals))) - 1))
+    return vals[k]
+
+
+def oracle_sample_check(w, nsample=4096, seed=9):
+    """Density at a random sample of voxels vs the FP64 NumPy oracle (normalised with the GPU's sum)."""
+    from oracle import density as od
+    from oracle.basisfile import read_basis
+    from oracle.system import make_system
+    sysm = make_system(w.mol.types, w.mol.coords, w.mol.dm, read_basis(os.path.join(PKG, "qmtools", "data",
+                                                                                     "cc-pvdz.bin")))
+    rng = numpy.random.default_rng(seed)
+    nx, ny, nz, h, o = w.grid.geometry()
+    idx = rng.choice(w.N, size=nsample, replace=False)
+    i, j, k = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    pts = numpy.asarray(o) + (numpy.stack([i, j, k], 1) + 0.5) * h
+    raw = 2.0 * od.rho_on_points(sysm, pts) * h ** 3
+    q = w.q64.cpu().numpy()
+    gpu_total = float(numpy.sum(q))   # = sum Z after normalisation
+    # reproduce the normalisation with the GPU's un-normalised total
+    nat = w.nat
+    tot = ctypes.c_double(0.0)
+    nat.call("qm_density", w.dm.handle, ctypes.c_void_p(w.stream.cuda_stream), ctypes.byref(w.geom), 1,
+             nat.QM_PREC_FP64, nat.QM_F64, nat.dptr(w.v64), 0, ctypes.byref(tot))
+    unnorm = w.v64.cpu().numpy()[idx]
+    err = float(numpy.abs(unnorm - raw).max() / numpy.abs(raw).max())
+    return err, gpu_total, tot.value
+
+
+def cpu_baseline(w, budget_s=20.0):
+    """FP64 NumPy oracle on the host cores, on bounded samples of the same workload."""
+    from oracle import density as od
+    from oracle import fft as off
+    from oracle import gto as og
+    from oracle.basisfile import read_basis
+    from oracle.system import make_system
+    cores = len(os.sched_getaffinity(0))
+    sysm = make_system(w.mol.types, w.mol.coords, w.mol.dm, read_basis(os.path.join(PKG, "qmtools", "data",
+                                                                                     "cc-pvdz.bin")))
+    nx, ny, nz, h, o = w.grid.geometry()
+    # density: a slab of whole x-rows at the molecule's centre plane
+    ns = 32768
+    rng = numpy.random.default_rng(5)
+    idx = rng.choice(w.N, size=ns, replace=False)
+    i, j, k = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    pts = numpy.asarray(o) + (numpy.stack([i, j, k], 1) + 0.5) * h
+    t0 = time.perf_counter()
+    od.rho_on_points(sysm, pts, chunk=8192)
+    t_dens = (time.perf_counter() - t0) * w.N / ns
+    # free-space FFT: the same solve on a 64^3 grid (128^3 padded), scaled by M log M to 512^3
+    m = 64
+    qs = numpy.random.default_rng(6).uniform(0, 1e-3, m ** 3)
+    t0 = time.perf_counter()
+    off.hartree_fft(qs, w.mol.types, w.mol.coords, m, m, m, h * nx / m, o, mode="freespace")
+    t_small = time.perf_counter() - t0
+    M_big, M_small = (2 * nx) ** 3, (2 * m) ** 3
+    t_fft = t_small * (M_big * math.log2(M_big)) / (M_small * math.log2(M_small))
+    # GTO: a few points, scaled to 10^4
+    kp = 2
+    t0 = time.perf_counter()
+    og.potential(sysm, w.pts_host[:kp])
+    t_gto = (time.perf_counter() - t0) * w.npoints / kp
+    total = t_dens + t_fft + t_gto
+    return {
+        "value": w.N / total,
+        "unit": "grid pts/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"NumPy FP64 oracle (OpenBLAS, {cores} threads): density on {ns} random voxels of 256^3 "
+                   f"extrapolated x{w.N // ns}; free-space FFT Poisson timed on 64^3 (128^3 padded) and scaled by "
+                   f"M log M to 512^3; GTO potential at {kp} points scaled to {w.npoints}; "
+                   f"seconds per step: density {t_dens:.1f}, FFT {t_fft:.1f}, GTO {t_gto:.1f} (extrapolated)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on one MI355X; "
+              "molecules and grid slabs shard across GPUs only by design)", file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", str(len(os.sched_getaffinity(0))))
+    import torch
+    from qmtools import _native as nat
+    log("building workload")
+    w = Workload(n=args.grid, precision=args.precision)
+    log(f"molecule: {int(w.mol.natoms)} atoms, nbf {w.mol.nbf}; grid {w.n}^3, step {float(w.grid.step):.5f} bohr")
+    other = "mixed" if args.precision == "fp64" else "fp64"
+
+    # main measurement
+    wall, lat = w.timed(args.steps, args.warmup, profile=True)
+    kid = nat.QM_K_DENSITY_CONTRACT if args.precision == "fp64" else nat.QM_K_DENSITY_SPLIT
+    k_ms, k_n = w.kernel_ms(kid)
+    phi_ms, phi_n = w.kernel_ms(nat.QM_K_PHI if args.precision == "fp64" else nat.QM_K_PHI_SPLIT)
+    fft_ms, fft_n = w.kernel_ms(nat.QM_K_FFT_AXIS)
+    gto_ms, gto_n = w.kernel_ms(nat.QM_K_GTO)
+    ms_step = wall * 1000.0 / args.steps
+    log(f"{args.precision}: {ms_step:.2f} ms/step; contraction {k_ms / max(k_n, 1):.3f} ms x {k_n}, "
+        f"phi {phi_ms / max(phi_n, 1):.3f} ms x {phi_n}, fft pass {fft_ms / max(fft_n, 1):.3f} ms x {fft_n}, "
+        f"gto {gto_ms / max(gto_n, 1):.3f} ms x {gto_n}")
+
+    # keep the results of this path, run the other path once for the comparison
+    q_main = (w.q64 if args.precision == "fp64" else w.q32).double().clone()
+    v_main = (w.v64 if args.precision == "fp64" else w.v32).double().clone()
+    wall_o, lat_o = w.timed(max(2, args.steps // 2), 1, precision=other)
+    q_other = (w.q64 if other == "fp64" else w.q32).double()
+    v_other = (w.v64 if other == "fp64" else w.v32).double()
+    q64, q_mx = (q_main, q_other) if args.precision == "fp64" else (q_other, q_main)
+    v64, v_mx = (v_main, v_other) if args.precision == "fp64" else (v_other, v_main)
+    rho_err = float((q_mx - q64).abs().max() / q64.abs().max())
+    chg_err = float(abs(q_mx.sum() - q64.sum()) / q64.sum())
+    vh_err = float((v_mx - v64).abs().max() / v64.abs().max())
+    w.q64.copy_(q64)
+
+    # roofline of the dominant kernel (the density contraction)
+    nbf = w.mol.nbf
+    launch_pts = w.N / max(k_n / args.steps, 1)
+    flops_per_launch = (2.0 * nbf * nbf + 2.0 * nbf) * launch_pts
+    mult = 1.0 if args.precision == "fp64" else 3.0     # hi/lo split executes 3 products
+    avg_ms = k_ms / max(k_n, 1)
+    achieved = flops_per_launch * mult / (avg_ms * 1e-3) / 1e12
+    peak = FP64_MFMA_PEAK_TFLOPS if args.precision == "fp64" else F16_MFMA_PEAK_TFLOPS
+
+    check = None
+    if not args.no_check:
+        log("oracle sample check")
+        err, gsum, raw = oracle_sample_check(w)
+        check = {"density_sample_rel_err_vs_numpy_fp64": err, "voxels": 4096, "normalised_sum": gsum,
+                 "raw_sum": raw}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(w)
+
+    value = w.N / (ms_step / 1000.0)
+    out = {
+        "metric": "grid pts/s density+Hartree at <=1e-6 rel err vs FP64; p50/p99 latency",
+        "value": value,
+        "unit": "grid pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "p50_ms": pct(lat, 50),
+        "p99_ms": pct(lat, 99),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f16x3/f32",
+        "data": "synthetic (seeded 60-atom molecule, P = 2 C C^T with QR-orthonormal C)",
+        "config": {"workload": "60-atom cc-pVDZ (606 bf), 256^3 grid: density + free-space FFT Poisson "
+                               "(512^3 padded) + GTO V_H at 1e4 points",
+                   "precision": args.precision, "grid": [w.n] * 3, "step_bohr": float(w.grid.step),
+                   "nbf": nbf, "points": w.npoints},
+        "roofline": {"bound": "mfma", "kernel": "density contraction (Phi P Phi^T diagonal)",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None,
+                     "algorithmic": f"(2 nbf^2 + 2 nbf) flops/point x {launch_pts:.0f} points/launch"
+                                    + (" x 3 split products" if mult > 1 else ""),
+                     "avg_ms": avg_ms, "launches": k_n},
+        "breakdown_ms_per_step": {"contraction": k_ms / args.steps, "phi": phi_ms / args.steps,
+                                  "fft_passes": fft_ms / args.steps, "gto": gto_ms / args.steps},
+        other + "_path": {"value": w.N / (wall_o / max(2, args.steps // 2)), "p50_ms": pct(lat_o, 50),
+                          "p99_ms": pct(lat_o, 99)},
+        "mixed_vs_fp64": {"rho_maxabs_rel": rho_err, "charge_rel": chg_err, "vh_fft_maxabs_rel": vh_err},
+        "oracle_check": check,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -202,6 +202,29 @@ int qm_hartree_gto_grid(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
 int qm_vne_classic(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int ndiv, int dtype,
                    void* out);
 
+/* ---------------------------------------------------------------- instrumentation */
+
+/* Kernel ids for qm_profile_read. */
+enum {
+  QM_K_DENSITY_CONTRACT = 0,   /* FP64 MFMA contraction */
+  QM_K_DENSITY_SPLIT = 1,      /* FP16/BF16 hi/lo MFMA contraction */
+  QM_K_PHI = 2,                /* basis values, FP64 */
+  QM_K_PHI_SPLIT = 3,          /* basis values, FP32 + 16-bit split */
+  QM_K_FFT_AXIS = 4,           /* one FFT pass along one axis */
+  QM_K_GTO = 5,                /* Hermite/Boys potential */
+  QM_K_JACOBI = 6,             /* one Jacobi sweep (FP64 or FP32) */
+  QM_K_COUNT = 7
+};
+
+/* No reference counterpart (the reference times nothing on the device).
+ * enable != 0 starts recording hipEvents around every launch of the kernels
+ * above, on the stream they are launched on, and clears earlier records;
+ * enable == 0 stops recording.  qm_profile_read synchronises on the recorded
+ * events and returns the summed duration (ms) and the launch count of one
+ * kernel id. */
+int qm_profile(qm_ctx* ctx, int enable);
+int qm_profile_read(qm_ctx* ctx, int kernel, double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

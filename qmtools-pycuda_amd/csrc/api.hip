@@ -60,6 +60,8 @@ int qm_check_geom(qm_ctx* ctx, const qm_geom* g) {
   return QM_OK;
 }
 
+static void clear_records(qm_ctx* ctx);
+
 extern "C" {
 
 int qm_version(void) { return QM_VERSION_MAJOR * 10000 + QM_VERSION_MINOR * 100 + QM_VERSION_PATCH; }
@@ -94,11 +96,50 @@ int qm_ctx_destroy(qm_ctx* ctx) {
     if (w.ptr) (void)hipFree(w.ptr);
   if (ctx->host_scalars) (void)hipHostFree(ctx->host_scalars);
   qm_fft_cache_free(ctx->fft);
+  clear_records(ctx);
   delete ctx;
   return QM_OK;
 }
 
 const char* qm_last_error(const qm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static void clear_records(qm_ctx* ctx) {
+  for (auto& r : ctx->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  ctx->recs.clear();
+}
+
+int qm_profile(qm_ctx* ctx, int enable) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  if (enable) {
+    QM_CHECK_HIP(ctx, hipDeviceSynchronize());
+    clear_records(ctx);
+  }
+  ctx->prof = enable != 0;
+  return QM_OK;
+}
+
+int qm_profile_read(qm_ctx* ctx, int kernel, double* total_ms, int64_t* launches) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, kernel >= 0 && kernel < QM_K_COUNT, QM_EINVAL, "profile: bad kernel id");
+  double tot = 0.0;
+  int64_t n = 0;
+  for (auto& r : ctx->recs) {
+    if (r.id != kernel) continue;
+    QM_CHECK_HIP(ctx, hipEventSynchronize(r.b));
+    float ms = 0.f;
+    QM_CHECK_HIP(ctx, hipEventElapsedTime(&ms, r.a, r.b));
+    tot += ms;
+    ++n;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = n;
+  return QM_OK;
+}
 
 static void free_mol_device(qm_mol* m) {
   if (m->atoms) (void)hipFree(m->atoms);

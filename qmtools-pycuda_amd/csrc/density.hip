@@ -16,7 +16,8 @@
 // (kernel_density.cu:74-80,138) and multiplies by 2 for the closed shell (:147).
 //
 // Layouts:
-//   FP64:   Phi^T[f * ld + t]            (f64 MFMA 16x16x4: A lane l = (t: l%16, k: l/16))
+//   FP64:   Phi^T[f * ld + t]            (f64 MFMA 16x16x4: A lane l = (t: l%16, k: l/16);
+//                                         D lane l, register r = (t: 4r + l/16, n: l%16))
 //   F16/BF16: K8-interleaved [(f/8) * ld + t] * 8 + f%8, hi and lo planes
 //           (MFMA 16x16x32: A lane l = (t: l%16, k: 8*(l/16)+0..7), one 16-byte load)
 //   P uses the same layouts with nbf_pad in place of ld (P is symmetric).
@@ -259,14 +260,15 @@ __global__ __launch_bounds__(kThreads) void k_contract_f64(const double* __restr
         acc[1][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b[cb], acc[1][cb], 0, 0, 0);
       }
     }
-    // epilogue: D[i = 4*lk + r][j = li]  ->  point row0 + rb*16 + i, column n0 + cb*16 + j
+    // epilogue: the f64 16x16x4 D layout is D[i = 4*r + lk][j = li] (lane l, register r)
+    //   ->  point row0 + rb*16 + i, column n0 + cb*16 + j
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
-        const double* ph = phi + (int64_t)(n0 + cb * 16 + li) * ld + row0 + rb * 16 + 4 * lk;
+        const double* ph = phi + (int64_t)(n0 + cb * 16 + li) * ld + row0 + rb * 16 + lk;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) part[rb][r] += acc[rb][cb][r] * ph[r];
+        for (int r = 0; r < 4; ++r) part[rb][r] += acc[rb][cb][r] * ph[4 * r];
       }
   }
 #pragma unroll
@@ -274,7 +276,7 @@ __global__ __launch_bounds__(kThreads) void k_contract_f64(const double* __restr
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const double v = sum16(part[rb][r]);
-      const int t = row0 + rb * 16 + 4 * lk + r;
+      const int t = row0 + rb * 16 + 4 * r + lk;
       if (li == 0 && t < npts) store_rho<TO>(out, v0 + t, v * scale);
     }
 }
@@ -466,9 +468,13 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
     const int nv = (int)std::min<int64_t>(nvox, nvox_tot - v0);
     const int npts = nv * nsub3;
     const int ld = (int)round_up(npts, kTileM);
-    hipLaunchKernelGGL(k_phi_f64, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s, mv, gv,
-                       v0, nv, nsub, npts, (const double*)nullptr, ld, np, (double*)phi);
+    {
+      ProfScope ps(ctx, s, QM_K_PHI);
+      hipLaunchKernelGGL(k_phi_f64, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s, mv, gv,
+                         v0, nv, nsub, npts, (const double*)nullptr, ld, np, (double*)phi);
+    }
     QM_CHECK_LAUNCH(ctx);
+    ProfScope pc(ctx, s, QM_K_DENSITY_CONTRACT);
     if (nsub3 == 1) {
       hipLaunchKernelGGL(k_contract_f64<TO>, dim3(ld / kTileM), dim3(kThreads), 0, s, (const double*)phi, ld,
                          (const double*)mol->P64, np, npts, h3, out, v0);
@@ -517,9 +523,13 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     const int ld = (int)round_up(npts, kTileM);
     T16* hi = (T16*)phi;
     T16* lo = hi + (int64_t)np * ld;
-    hipLaunchKernelGGL(k_phi_split<T16>, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s,
-                       mv, gv, v0, nv, nsub, npts, ld, np, hi, lo, pscale);
+    {
+      ProfScope ps(ctx, s, QM_K_PHI_SPLIT);
+      hipLaunchKernelGGL(k_phi_split<T16>, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s,
+                         mv, gv, v0, nv, nsub, npts, ld, np, hi, lo, pscale);
+    }
     QM_CHECK_LAUNCH(ctx);
+    ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
     if (nsub3 == 1) {
       hipLaunchKernelGGL((k_contract_split<T16, TO>), dim3(ld / kTileM), dim3(kThreads), 0, s, (const T16*)hi,
                          (const T16*)lo, ld, (const float*)pscale, Phi, Plo, np, npts, h3, out, v0);

@@ -437,6 +437,7 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
                                           hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
     attr_set = true;
   }
+  ProfScope ps(ctx, s, QM_K_FFT_AXIS);
   hipLaunchKernelGGL((k_fft_axis<R, INV>), dim3((unsigned)nwg), dim3(kThreads), shmem, s, data, tw, plan, ln,
                      inplace ? 1 : 0);
   QM_CHECK_LAUNCH(ctx);

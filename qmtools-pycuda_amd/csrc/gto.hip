@@ -424,8 +424,11 @@ int gto_eval(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const double* pts, const G
   double* partial = nullptr;
   if ((rc = qm_ws(ctx, WS_GTO, sizeof(double) * (size_t)nchunk * npts, (void**)&partial))) return rc;
   QM_REQUIRE(ctx, pblocks <= 0x7fffffff, QM_ELIMIT, "gto: too many points");
-  hipLaunchKernelGGL(k_gto, dim3((unsigned)pblocks, nchunk), dim3(kThreads), 0, s, dl, (const double*)c->boys,
+  {
+    ProfScope ps(ctx, s, QM_K_GTO);
+    hipLaunchKernelGGL(k_gto, dim3((unsigned)pblocks, nchunk), dim3(kThreads), 0, s, dl, (const double*)c->boys,
                      pts, gv, npts, nchunk, partial);
+  }
   QM_CHECK_LAUNCH(ctx);
   hipLaunchKernelGGL(k_gto_finish<TO>, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s,
                      (const double*)partial, nchunk, (const double4*)mol->atoms, mol->natoms, pts, gv, npts, out);

@@ -167,6 +167,7 @@ dim3 sweep_grid(const qm_geom* g) {
 
 template <typename T>
 int sweep(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const T* rho, const T* a, T* b) {
+  ProfScope ps(ctx, s, QM_K_JACOBI);
   hipLaunchKernelGGL(k_sweep<T>, sweep_grid(g), dim3(kBX, kBY), 0, s, rho, a, b, g->nx, g->ny, g->nz,
                      (T)g->step);
   QM_CHECK_LAUNCH(ctx);

@@ -68,12 +68,20 @@ enum WsSlot {
 
 struct FftPlanCache;
 
+// kernel timing records (qm_profile / qm_profile_read)
+struct ProfRec {
+  int id;
+  hipEvent_t a, b;
+};
+
 struct qm_ctx {
   int device = 0;
   std::string err;
   Workspace ws[WS_COUNT];
   double* host_scalars = nullptr;   // pinned host buffer for results
   FftPlanCache* fft = nullptr;
+  bool prof = false;
+  std::vector<ProfRec> recs;
 };
 
 struct qm_mol {
@@ -137,3 +145,21 @@ int qm_scale_dev(qm_ctx* ctx, hipStream_t s, int dtype, void* a, int64_t n, cons
 int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol);
 void qm_fft_cache_free(FftPlanCache* c);
 void qm_gto_cache_free(void* c);
+
+// Brackets one kernel launch with events on its stream when profiling is on.
+struct ProfScope {
+  qm_ctx* ctx;
+  hipStream_t s;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(qm_ctx* c, hipStream_t st, int kid) : ctx(c), s(st), id(kid) {
+    if (ctx->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, s);
+  }
+  ~ProfScope() {
+    if (a && b) {
+      (void)hipEventRecord(b, s);
+      ctx->recs.push_back({id, a, b});
+    }
+  }
+};

@@ -70,7 +70,10 @@ SIGNATURES = {
     "qm_hartree_gto_points": [_vp, _vp, _vp, _vp, _i64, _vp],
     "qm_hartree_gto_grid": [_vp, _vp, _vp, _pg, _i, _vp],
     "qm_vne_classic": [_vp, _vp, _vp, _pg, _i, _i, _vp],
+    "qm_profile": [_vp, _i],
+    "qm_profile_read": [_vp, _i, _pd, _c.POINTER(_i64)],
 }
+QM_K_DENSITY_CONTRACT, QM_K_DENSITY_SPLIT, QM_K_PHI, QM_K_PHI_SPLIT, QM_K_FFT_AXIS, QM_K_GTO, QM_K_JACOBI = range(7)
 _RESTYPE = {"qm_last_error": _c.c_char_p}
 
 _lock = threading.Lock()

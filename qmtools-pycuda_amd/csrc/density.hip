@@ -37,7 +37,6 @@ typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 
 constexpr int kPhiThreads = 256;
 constexpr int kTileM = 128;      // points per contraction workgroup (4 waves x 32 rows)
-constexpr int kTileN = 64;       // columns per n-block
 constexpr int kThreads = 256;
 
 // Real solid harmonic prefactors of the reference (kernel_density.cu:17-28).
@@ -139,25 +138,21 @@ __global__ __launch_bounds__(kPhiThreads) void k_phi_split(MolView m, GeomView g
   const int t = blockIdx.x * kPhiThreads + threadIdx.x;
   if (t >= ld) return;
   const bool live = t < npts;
-  float x = 0.f, y = 0.f, z = 0.f;
-  if (live) {
-    double xd, yd, zd;
-    chunk_point(g, v0, nvox, nsub, t, xd, yd, zd);
-    x = (float)xd;
-    y = (float)yd;
-    z = (float)zd;
-  }
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (live) chunk_point(g, v0, nvox, nsub, t, x, y, z);
+  // offsets to the nucleus are formed in FP64 and rounded once: rounding x and R
+  // separately to FP32 costs ~1e-6 bohr, i.e. ~1e-4 relative error in tight primitives
   float vmax = 0.f;
   if (live) {
     for (int s = 0; s < m.nshell; ++s) {
       const DevShell sh = m.shells[s];
       const double4 a = m.atoms[sh.atom];
-      const float dx = x - (float)a.x, dy = y - (float)a.y, dz = z - (float)a.z;
+      const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
       const float r2 = dx * dx + dy * dy + dz * dz;
       float radial = 0.f;
       for (int p = 0; p < sh.nprim; ++p) {
         const double2 pc = m.prims[sh.poff + p];
-        radial += (float)pc.y * __expf(-(float)pc.x * r2);
+        radial += (float)pc.y * expf(-(float)pc.x * r2);
       }
       float v[5];
       const int nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
@@ -174,12 +169,12 @@ __global__ __launch_bounds__(kPhiThreads) void k_phi_split(MolView m, GeomView g
     int nc = 2 * sh.L + 1;
     if (live) {
       const double4 a = m.atoms[sh.atom];
-      const float dx = x - (float)a.x, dy = y - (float)a.y, dz = z - (float)a.z;
+      const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
       const float r2 = dx * dx + dy * dy + dz * dz;
       float radial = 0.f;
       for (int p = 0; p < sh.nprim; ++p) {
         const double2 pc = m.prims[sh.poff + p];
-        radial += (float)pc.y * __expf(-(float)pc.x * r2);
+        radial += (float)pc.y * expf(-(float)pc.x * r2);
       }
       nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
     }
@@ -229,56 +224,118 @@ __device__ __forceinline__ float sum16f(float v) {
 template <typename TO>
 __device__ __forceinline__ void store_rho(TO* out, int64_t idx, double v) { out[idx] = (TO)v; }
 
-// rho_t * scale for the 128 points of this workgroup; MFMA f64 16x16x4.
-// out_direct: rows map to out[v0 + t] (subgrid 1); else rows go to rho_tmp[t].
-template <typename TO>
-__global__ __launch_bounds__(kThreads) void k_contract_f64(const double* __restrict__ phi, int ld,
-                                                           const double* __restrict__ P, int nbf_pad, int npts,
-                                                           double scale, TO* __restrict__ out, int64_t v0) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int row0 = blockIdx.x * kTileM + wave * 32;
+// ---------------------------------------------------------------------------
+// Contraction kernels.  A workgroup (8 waves) owns TM points and every column
+// of one column pass; the accumulators of X = Phi P for all its columns live
+// in registers, so the Phi tile is read from HBM once for the GEMM (staged in
+// double-buffered LDS, shared by the 8 waves) and once more for the row-dot
+// epilogue.  P (L2-resident) is streamed straight into MFMA operands; wave w
+// owns columns [w NT 16, (w+1) NT 16) of the pass.  Per-row partial sums of
+// each wave go to LDS and are added in a fixed order (deterministic).
+constexpr int kWaves = 8;
+constexpr int kWG = kWaves * 64;
+
+// FP64: MFMA f64 16x16x4.  A lane l = (t: l%16, k: l/16), B lane l = (k: l/16, n: l%16),
+// D lane l, register r = (t: 4r + l/16, n: l%16).
+template <int NT, typename TO>
+__global__ __launch_bounds__(kWG) void k_contract_f64(const double* __restrict__ phi, int ld,
+                                                      const double* __restrict__ P, int nbf_pad, int npts,
+                                                      double scale, TO* __restrict__ out, int64_t v0) {
+  constexpr int TM = 64, KS = 16, PAD = 16;
+  __shared__ double As[2][KS][TM + PAD];
+  __shared__ double red[kWaves][TM];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  double part[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-  for (int n0 = 0; n0 < nbf_pad; n0 += kTileN) {
-    d4 acc[2][4];
+  const int row0 = blockIdx.x * TM;
+  for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
+  const int nks = nbf_pad / KS;
+  for (int c0 = 0; c0 < nbf_pad; c0 += kWaves * NT * 16) {
+    const int wc0 = c0 + wave * NT * 16;
+    d4 acc[4][NT];
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = d4{0, 0, 0, 0};
-    const double* pa = phi + (int64_t)lk * ld + row0 + li;
-    const double* pb = P + (int64_t)lk * nbf_pad + n0 + li;
-    for (int k0 = 0; k0 < nbf_pad; k0 += 4) {
-      const double a0 = pa[(int64_t)k0 * ld];
-      const double a1 = pa[(int64_t)k0 * ld + 16];
-      double b[4];
+      for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = d4{0, 0, 0, 0};
+    // stage 0
+    double pre[2];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) b[cb] = pb[(int64_t)k0 * nbf_pad + cb * 16];
+    for (int q = 0; q < 2; ++q) {
+      const int e = threadIdx.x + q * kWG, kk = e / TM, t = e % TM;
+      pre[q] = phi[(int64_t)kk * ld + row0 + t];
+    }
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        acc[0][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b[cb], acc[0][cb], 0, 0, 0);
-        acc[1][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b[cb], acc[1][cb], 0, 0, 0);
+    for (int q = 0; q < 2; ++q) {
+      const int e = threadIdx.x + q * kWG;
+      As[0][e / TM][e % TM] = pre[q];
+    }
+    __syncthreads();
+    for (int s = 0; s < nks; ++s) {
+      const int k0 = s * KS;
+      const bool more = s + 1 < nks;
+      if (more) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int e = threadIdx.x + q * kWG, kk = e / TM, t = e % TM;
+          pre[q] = phi[(int64_t)(k0 + KS + kk) * ld + row0 + t];
+        }
+      }
+      if (wc0 < nbf_pad) {
+        const double(*A)[TM + PAD] = As[s & 1];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks += 4) {
+          double b[NT];
+#pragma unroll
+          for (int ct = 0; ct < NT; ++ct) {
+            const int n = wc0 + ct * 16 + li;
+            b[ct] = n < nbf_pad ? P[(int64_t)(k0 + ks + lk) * nbf_pad + n] : 0.0;
+          }
+#pragma unroll
+          for (int rt = 0; rt < 4; ++rt) {
+            const double a = A[ks + lk][rt * 16 + li];
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct)
+              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[ct], acc[rt][ct], 0, 0, 0);
+          }
+        }
+      }
+      if (more) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int e = threadIdx.x + q * kWG;
+          As[(s + 1) & 1][e / TM][e % TM] = pre[q];
+        }
+      }
+      __syncthreads();
+    }
+    if (wc0 < nbf_pad) {
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        double part[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+          const int n = wc0 + ct * 16 + li;
+          if (n < nbf_pad) {
+            const double* ph = phi + (int64_t)n * ld + row0 + rt * 16 + lk;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) part[r] += acc[rt][ct][r] * ph[4 * r];
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double v = sum16(part[r]);
+          if (li == 0) red[wave][rt * 16 + 4 * r + lk] += v;
+        }
       }
     }
-    // epilogue: the f64 16x16x4 D layout is D[i = 4*r + lk][j = li] (lane l, register r)
-    //   ->  point row0 + rb*16 + i, column n0 + cb*16 + j
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        const double* ph = phi + (int64_t)(n0 + cb * 16 + li) * ld + row0 + rb * 16 + lk;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) part[rb][r] += acc[rb][cb][r] * ph[4 * r];
-      }
   }
+  __syncthreads();
+  if (threadIdx.x < TM) {
+    double v = 0.0;
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double v = sum16(part[rb][r]);
-      const int t = row0 + rb * 16 + 4 * r + lk;
-      if (li == 0 && t < npts) store_rho<TO>(out, v0 + t, v * scale);
-    }
+    for (int w = 0; w < kWaves; ++w) v += red[w][threadIdx.x];
+    const int t = row0 + threadIdx.x;
+    if (t < npts) store_rho<TO>(out, v0 + t, v * scale);
+  }
 }
 
 template <typename T16>
@@ -298,73 +355,111 @@ struct Vec8<__bf16> {
   }
 };
 
-// Split-precision contraction: X = Ahi Bhi + Ahi Blo + Alo Bhi (FP32 accumulate),
-// rho_t = pscale_t * pfac * sum_n (Ahi + Alo)[t][n] X[t][n].
-template <typename T16, typename TO>
-__global__ __launch_bounds__(kThreads) void k_contract_split(const T16* __restrict__ phi_hi,
-                                                             const T16* __restrict__ phi_lo, int ld,
-                                                             const float* __restrict__ pscale,
-                                                             const T16* __restrict__ p_hi,
-                                                             const T16* __restrict__ p_lo, int nbf_pad,
-                                                             int npts, double scale, TO* __restrict__ out,
-                                                             int64_t v0) {
+// Split precision: X = Ahi Bhi + Ahi Blo + Alo Bhi (FP32 accumulate),
+// rho_t = pscale_t * scale * sum_n (Ahi + Alo)[t][n] X[t][n] (row-dot in FP64).
+// MFMA 16x16x32: A lane l = (t: l%16, k: 8(l/16)+0..7), B likewise, D lane l, reg r = (t: 4(l/16)+r, n: l%16).
+template <typename T16, int NT, typename TO>
+__global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ phi_hi,
+                                                        const T16* __restrict__ phi_lo, int ld,
+                                                        const float* __restrict__ pscale,
+                                                        const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
+                                                        int nbf_pad, int npts, double scale, TO* __restrict__ out,
+                                                        int64_t v0) {
   typedef typename Vec8<T16>::type V8;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int row0 = blockIdx.x * kTileM + wave * 32;
+  constexpr int TM = 128, KG = 4;   // 4 groups of 8 k = 32 k per stage
+  __shared__ uint4 Ah[2][KG][TM], Al[2][KG][TM];
+  __shared__ double red[kWaves][TM];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  float part[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-  for (int n0 = 0; n0 < nbf_pad; n0 += kTileN) {
-    f4 acc[2][4];
+  const int row0 = blockIdx.x * TM;
+  for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
+  const int nks = nbf_pad / 32;
+  const int sg = threadIdx.x / TM, st = threadIdx.x % TM;   // staging: one uint4 per plane per thread
+  for (int c0 = 0; c0 < nbf_pad; c0 += kWaves * NT * 16) {
+    const int wc0 = c0 + wave * NT * 16;
+    f4 acc[8][NT];
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int rt = 0; rt < 8; ++rt)
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f4{0, 0, 0, 0};
-    for (int k0 = 0; k0 < nbf_pad; k0 += 32) {
-      const int64_t kg = (k0 >> 3) + lk;
-      V8 ah[2], al[2], bh[4], bl[4];
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
-        const int64_t o = (kg * ld + row0 + rb * 16 + li) * 8;
-        ah[rb] = *(const V8*)(phi_hi + o);
-        al[rb] = *(const V8*)(phi_lo + o);
+      for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0, 0, 0, 0};
+    uint4 ph = *(const uint4*)(phi_hi + ((int64_t)sg * ld + row0 + st) * 8);
+    uint4 pl = *(const uint4*)(phi_lo + ((int64_t)sg * ld + row0 + st) * 8);
+    Ah[0][sg][st] = ph;
+    Al[0][sg][st] = pl;
+    __syncthreads();
+    for (int s = 0; s < nks; ++s) {
+      const bool more = s + 1 < nks;
+      if (more) {
+        const int64_t o = ((int64_t)((s + 1) * KG + sg) * ld + row0 + st) * 8;
+        ph = *(const uint4*)(phi_hi + o);
+        pl = *(const uint4*)(phi_lo + o);
       }
+      if (wc0 < nbf_pad) {
+        V8 bh[NT], bl[NT];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        const int64_t o = (kg * nbf_pad + n0 + cb * 16 + li) * 8;
-        bh[cb] = *(const V8*)(p_hi + o);
-        bl[cb] = *(const V8*)(p_lo + o);
-      }
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-          acc[rb][cb] = Vec8<T16>::mma(al[rb], bh[cb], acc[rb][cb]);
-          acc[rb][cb] = Vec8<T16>::mma(ah[rb], bl[cb], acc[rb][cb]);
-          acc[rb][cb] = Vec8<T16>::mma(ah[rb], bh[cb], acc[rb][cb]);
+        for (int ct = 0; ct < NT; ++ct) {
+          const int n = wc0 + ct * 16 + li;
+          if (n < nbf_pad) {
+            const int64_t o = ((int64_t)(s * KG + lk) * nbf_pad + n) * 8;
+            bh[ct] = *(const V8*)(p_hi + o);
+            bl[ct] = *(const V8*)(p_lo + o);
+          } else {
+            bh[ct] = V8{};
+            bl[ct] = V8{};
+          }
         }
+#pragma unroll
+        for (int rt = 0; rt < 8; ++rt) {
+          const uint4 uh = Ah[s & 1][lk][rt * 16 + li];
+          const uint4 ul = Al[s & 1][lk][rt * 16 + li];
+          const V8 ah = *(const V8*)&uh;
+          const V8 al = *(const V8*)&ul;
+#pragma unroll
+          for (int ct = 0; ct < NT; ++ct) {
+            acc[rt][ct] = Vec8<T16>::mma(al, bh[ct], acc[rt][ct]);
+            acc[rt][ct] = Vec8<T16>::mma(ah, bl[ct], acc[rt][ct]);
+            acc[rt][ct] = Vec8<T16>::mma(ah, bh[ct], acc[rt][ct]);
+          }
+        }
+      }
+      if (more) {
+        Ah[(s + 1) & 1][sg][st] = ph;
+        Al[(s + 1) & 1][sg][st] = pl;
+      }
+      __syncthreads();
     }
+    if (wc0 < nbf_pad) {
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+      for (int rt = 0; rt < 8; ++rt) {
+        double part[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        const int n = n0 + cb * 16 + li;
-        const int64_t base = ((int64_t)(n >> 3) * ld + row0 + rb * 16 + 4 * lk) * 8 + (n & 7);
+        for (int ct = 0; ct < NT; ++ct) {
+          const int n = wc0 + ct * 16 + li;
+          if (n < nbf_pad) {
+            const int64_t base = ((int64_t)(n >> 3) * ld + row0 + rt * 16 + 4 * lk) * 8 + (n & 7);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const float v = (float)phi_hi[base + 8 * r] + (float)phi_lo[base + 8 * r];
+              part[r] += (double)(acc[rt][ct][r] * v);
+            }
+          }
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const float ph = (float)phi_hi[base + 8 * r] + (float)phi_lo[base + 8 * r];
-          part[rb][r] += acc[rb][cb][r] * ph;
+          const double v = sum16(part[r]);
+          if (li == 0) red[wave][rt * 16 + 4 * lk + r] += v;
         }
       }
-  }
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const float v = sum16f(part[rb][r]);
-      const int t = row0 + rb * 16 + 4 * lk + r;
-      if (li == 0 && t < npts) store_rho<TO>(out, v0 + t, (double)v * (double)pscale[t] * scale);
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < TM) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) v += red[w][threadIdx.x];
+    const int t = row0 + threadIdx.x;
+    if (t < npts) store_rho<TO>(out, v0 + t, v * (double)pscale[t] * scale);
+  }
 }
 
 // Sum the nsub^3 sub-point densities of each voxel: out[v0 + v] = sum_s rho[s * nvox + v].
@@ -442,6 +537,39 @@ int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol) {
 
 namespace {
 
+// columns per wave per pass: NT tiles of 16, NT = min(5, ceil(nbf_pad / 128))
+inline int pick_nt(int np) { return std::min(5, (np + 127) / 128); }
+
+template <typename TO>
+void launch_contract_f64(hipStream_t s, int ld, const double* phi, const double* P, int np, int npts, double scale,
+                         TO* out, int64_t v0) {
+  const dim3 grid(ld / 64), block(kWG);
+  switch (pick_nt(np)) {
+    case 1: hipLaunchKernelGGL((k_contract_f64<1, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
+    case 2: hipLaunchKernelGGL((k_contract_f64<2, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
+    case 3: hipLaunchKernelGGL((k_contract_f64<3, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
+    case 4: hipLaunchKernelGGL((k_contract_f64<4, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
+    default: hipLaunchKernelGGL((k_contract_f64<5, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
+  }
+}
+
+template <typename T16, typename TO>
+void launch_contract_split(hipStream_t s, int ld, const T16* hi, const T16* lo, const float* pscale, const T16* Ph,
+                           const T16* Pl, int np, int npts, double scale, TO* out, int64_t v0) {
+  const dim3 grid(ld / 128), block(kWG);
+#define QM_SPLIT_CASE(NT)                                                                                    \
+  hipLaunchKernelGGL((k_contract_split<T16, NT, TO>), grid, block, 0, s, hi, lo, ld, pscale, Ph, Pl, np, npts, \
+                     scale, out, v0)
+  switch (pick_nt(np)) {
+    case 1: QM_SPLIT_CASE(1); break;
+    case 2: QM_SPLIT_CASE(2); break;
+    case 3: QM_SPLIT_CASE(3); break;
+    case 4: QM_SPLIT_CASE(4); break;
+    default: QM_SPLIT_CASE(5); break;
+  }
+#undef QM_SPLIT_CASE
+}
+
 template <typename TO>
 int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int nsub, TO* out) {
   const int np = mol->nbf_pad;
@@ -476,11 +604,10 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
     QM_CHECK_LAUNCH(ctx);
     ProfScope pc(ctx, s, QM_K_DENSITY_CONTRACT);
     if (nsub3 == 1) {
-      hipLaunchKernelGGL(k_contract_f64<TO>, dim3(ld / kTileM), dim3(kThreads), 0, s, (const double*)phi, ld,
-                         (const double*)mol->P64, np, npts, h3, out, v0);
+      launch_contract_f64<TO>(s, ld, (const double*)phi, (const double*)mol->P64, np, npts, h3, out, v0);
     } else {
-      hipLaunchKernelGGL(k_contract_f64<double>, dim3(ld / kTileM), dim3(kThreads), 0, s, (const double*)phi,
-                         ld, (const double*)mol->P64, np, npts, h3 / nsub3, rho, (int64_t)0);
+      launch_contract_f64<double>(s, ld, (const double*)phi, (const double*)mol->P64, np, npts, h3 / nsub3, rho,
+                                  (int64_t)0);
       QM_CHECK_LAUNCH(ctx);
       hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
                          nsub3, out, v0);
@@ -531,12 +658,9 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     QM_CHECK_LAUNCH(ctx);
     ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
     if (nsub3 == 1) {
-      hipLaunchKernelGGL((k_contract_split<T16, TO>), dim3(ld / kTileM), dim3(kThreads), 0, s, (const T16*)hi,
-                         (const T16*)lo, ld, (const float*)pscale, Phi, Plo, np, npts, h3, out, v0);
+      launch_contract_split<T16, TO>(s, ld, hi, lo, pscale, Phi, Plo, np, npts, h3, out, v0);
     } else {
-      hipLaunchKernelGGL((k_contract_split<T16, double>), dim3(ld / kTileM), dim3(kThreads), 0, s,
-                         (const T16*)hi, (const T16*)lo, ld, (const float*)pscale, Phi, Plo, np, npts,
-                         h3 / nsub3, rho, (int64_t)0);
+      launch_contract_split<T16, double>(s, ld, hi, lo, pscale, Phi, Plo, np, npts, h3 / nsub3, rho, (int64_t)0);
       QM_CHECK_LAUNCH(ctx);
       hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
                          nsub3, out, v0);

@@ -235,6 +235,9 @@ def main():
     w = Workload(n=args.grid, precision=args.precision)
     log(f"molecule: {int(w.mol.natoms)} atoms, nbf {w.mol.nbf}; grid {w.n}^3, step {float(w.grid.step):.5f} bohr")
     other = "mixed" if args.precision == "fp64" else "fp64"
+    from qmtools import autotune
+    tune = autotune.autotune_density(w.mol, w.grid, "mixed", use_cache=False)
+    log(f"autotune (mixed contraction): {tune}")
 
     # main measurement
     wall, lat = w.timed(args.steps, args.warmup, profile=True)
@@ -263,6 +266,17 @@ def main():
                    "vh_fft_maxabs_rel": float((v_mx - v64).abs().max() / v64.abs().max())}
         other_out = {"value": w.N / (wall_o / max(2, args.steps // 2)), "p50_ms": pct(lat_o, 50),
                      "p99_ms": pct(lat_o, 99)}
+        if other == "mixed":
+            wall_m, _ = w.timed(2, 0, precision="mixed", profile=True)
+            km, kn = w.kernel_ms(nat.QM_K_DENSITY_SPLIT)
+            pm, _ = w.kernel_ms(nat.QM_K_PHI_SPLIT)
+            fm, _ = w.kernel_ms(nat.QM_K_FFT_AXIS)
+            gm, _ = w.kernel_ms(nat.QM_K_GTO)
+            other_out["breakdown_ms_per_step"] = {"contraction": km / 2, "phi": pm / 2, "fft_passes": fm / 2,
+                                                  "gto": gm / 2}
+            other_out["contraction_avg_ms"] = km / max(kn, 1)
+            other_out["contraction_launches_per_step"] = kn / 2
+            log(f"mixed breakdown: {other_out['breakdown_ms_per_step']}")
         w.q64.copy_(q64)
 
     # roofline of the dominant kernel (the density contraction)
@@ -316,6 +330,7 @@ def main():
         other + "_path": other_out,
         "mixed_vs_fp64": err_out,
         "oracle_check": check,
+        "autotune": tune,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

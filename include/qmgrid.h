@@ -202,6 +202,23 @@ int qm_hartree_gto_grid(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
 int qm_vne_classic(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int ndiv, int dtype,
                    void* out);
 
+/* ---------------------------------------------------------------- tuning */
+
+/* Tile-shape knobs of the density contraction (no reference counterpart). */
+enum {
+  QM_TUNE_SPLIT_ROWS = 0,   /* points per workgroup of the 16-bit split contraction: 64 or 128 */
+  QM_TUNE_COUNT = 1
+};
+int qm_tune_set(qm_ctx* ctx, int knob, int value);
+int qm_tune_get(qm_ctx* ctx, int knob, int* value);
+
+/* Autotuner: times every candidate value of the knobs that apply to
+ * `precision` on a slab of grid `g` (at most ~1.6M points, written to a
+ * library workspace, not to any caller buffer), keeps the fastest in the
+ * context and returns its time (ms, host) in best_ms.  Synchronises `stream`. */
+int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int precision,
+                        double* best_ms);
+
 /* ---------------------------------------------------------------- instrumentation */
 
 /* Kernel ids for qm_profile_read. */

@@ -268,6 +268,12 @@ __global__ __launch_bounds__(kWG) void k_contract_f64(const double* __restrict__
       const int e = threadIdx.x + q * kWG;
       As[0][e / TM][e % TM] = pre[q];
     }
+    double bq[NT];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) {
+      const int n = wc0 + ct * 16 + li;
+      bq[ct] = n < nbf_pad ? P[(int64_t)lk * nbf_pad + n] : 0.0;
+    }
     __syncthreads();
     for (int s = 0; s < nks; ++s) {
       const int k0 = s * KS;
@@ -283,19 +289,23 @@ __global__ __launch_bounds__(kWG) void k_contract_f64(const double* __restrict__
         const double(*A)[TM + PAD] = As[s & 1];
 #pragma unroll
         for (int ks = 0; ks < KS; ks += 4) {
-          double b[NT];
+          // prefetch the P operands of the next k-substep (possibly in the next stage)
+          const int kn = k0 + ks + 4;
+          double bn[NT];
 #pragma unroll
           for (int ct = 0; ct < NT; ++ct) {
             const int n = wc0 + ct * 16 + li;
-            b[ct] = n < nbf_pad ? P[(int64_t)(k0 + ks + lk) * nbf_pad + n] : 0.0;
+            bn[ct] = (n < nbf_pad && kn < nbf_pad) ? P[(int64_t)(kn + lk) * nbf_pad + n] : 0.0;
           }
 #pragma unroll
           for (int rt = 0; rt < 4; ++rt) {
             const double a = A[ks + lk][rt * 16 + li];
 #pragma unroll
             for (int ct = 0; ct < NT; ++ct)
-              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[ct], acc[rt][ct], 0, 0, 0);
+              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bq[ct], acc[rt][ct], 0, 0, 0);
           }
+#pragma unroll
+          for (int ct = 0; ct < NT; ++ct) bq[ct] = bn[ct];
         }
       }
       if (more) {
@@ -358,7 +368,7 @@ struct Vec8<__bf16> {
 // Split precision: X = Ahi Bhi + Ahi Blo + Alo Bhi (FP32 accumulate),
 // rho_t = pscale_t * scale * sum_n (Ahi + Alo)[t][n] X[t][n] (row-dot in FP64).
 // MFMA 16x16x32: A lane l = (t: l%16, k: 8(l/16)+0..7), B likewise, D lane l, reg r = (t: 4(l/16)+r, n: l%16).
-template <typename T16, int NT, typename TO>
+template <typename T16, int NT, int TM, typename TO>
 __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ phi_hi,
                                                         const T16* __restrict__ phi_lo, int ld,
                                                         const float* __restrict__ pscale,
@@ -366,7 +376,9 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
                                                         int nbf_pad, int npts, double scale, TO* __restrict__ out,
                                                         int64_t v0) {
   typedef typename Vec8<T16>::type V8;
-  constexpr int TM = 128, KG = 4;   // 4 groups of 8 k = 32 k per stage
+  constexpr int KG = 4;             // 4 groups of 8 k = 32 k per stage
+  constexpr int RT = TM / 16;       // row tiles
+  constexpr int SPT = KG * TM / kWG; // staged uint4 per plane per thread (1 for TM 128)
   __shared__ uint4 Ah[2][KG][TM], Al[2][KG][TM];
   __shared__ double red[kWaves][TM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -374,22 +386,28 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
   const int row0 = blockIdx.x * TM;
   for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
   const int nks = nbf_pad / 32;
-  const int sg = threadIdx.x / TM, st = threadIdx.x % TM;   // staging: one uint4 per plane per thread
+  // staging: KG * TM uint4 per plane; threads below KG * TM load one each
+  const bool stager = threadIdx.x < KG * TM;
+  const int sg = threadIdx.x / TM, st = threadIdx.x % TM;
+  (void)SPT;
   for (int c0 = 0; c0 < nbf_pad; c0 += kWaves * NT * 16) {
     const int wc0 = c0 + wave * NT * 16;
-    f4 acc[8][NT];
+    f4 acc[RT][NT];
 #pragma unroll
-    for (int rt = 0; rt < 8; ++rt)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0, 0, 0, 0};
-    uint4 ph = *(const uint4*)(phi_hi + ((int64_t)sg * ld + row0 + st) * 8);
-    uint4 pl = *(const uint4*)(phi_lo + ((int64_t)sg * ld + row0 + st) * 8);
-    Ah[0][sg][st] = ph;
-    Al[0][sg][st] = pl;
+    uint4 ph{}, pl{};
+    if (stager) {
+      ph = *(const uint4*)(phi_hi + ((int64_t)sg * ld + row0 + st) * 8);
+      pl = *(const uint4*)(phi_lo + ((int64_t)sg * ld + row0 + st) * 8);
+      Ah[0][sg][st] = ph;
+      Al[0][sg][st] = pl;
+    }
     __syncthreads();
     for (int s = 0; s < nks; ++s) {
       const bool more = s + 1 < nks;
-      if (more) {
+      if (more && stager) {
         const int64_t o = ((int64_t)((s + 1) * KG + sg) * ld + row0 + st) * 8;
         ph = *(const uint4*)(phi_hi + o);
         pl = *(const uint4*)(phi_lo + o);
@@ -409,7 +427,7 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
           }
         }
 #pragma unroll
-        for (int rt = 0; rt < 8; ++rt) {
+        for (int rt = 0; rt < RT; ++rt) {
           const uint4 uh = Ah[s & 1][lk][rt * 16 + li];
           const uint4 ul = Al[s & 1][lk][rt * 16 + li];
           const V8 ah = *(const V8*)&uh;
@@ -422,7 +440,7 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
           }
         }
       }
-      if (more) {
+      if (more && stager) {
         Ah[(s + 1) & 1][sg][st] = ph;
         Al[(s + 1) & 1][sg][st] = pl;
       }
@@ -430,7 +448,7 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
     }
     if (wc0 < nbf_pad) {
 #pragma unroll
-      for (int rt = 0; rt < 8; ++rt) {
+      for (int rt = 0; rt < RT; ++rt) {
         double part[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
@@ -555,11 +573,15 @@ void launch_contract_f64(hipStream_t s, int ld, const double* phi, const double*
 
 template <typename T16, typename TO>
 void launch_contract_split(hipStream_t s, int ld, const T16* hi, const T16* lo, const float* pscale, const T16* Ph,
-                           const T16* Pl, int np, int npts, double scale, TO* out, int64_t v0) {
-  const dim3 grid(ld / 128), block(kWG);
-#define QM_SPLIT_CASE(NT)                                                                                    \
-  hipLaunchKernelGGL((k_contract_split<T16, NT, TO>), grid, block, 0, s, hi, lo, ld, pscale, Ph, Pl, np, npts, \
-                     scale, out, v0)
+                           const T16* Pl, int np, int npts, double scale, TO* out, int64_t v0, int tm) {
+  const dim3 grid(ld / tm), block(kWG);
+#define QM_SPLIT_CASE(NT)                                                                                        \
+  if (tm == 64)                                                                                                  \
+    hipLaunchKernelGGL((k_contract_split<T16, NT, 64, TO>), grid, block, 0, s, hi, lo, ld, pscale, Ph, Pl, np, npts, \
+                       scale, out, v0);                                                                          \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_contract_split<T16, NT, 128, TO>), grid, block, 0, s, hi, lo, ld, pscale, Ph, Pl, np,     \
+                       npts, scale, out, v0)
   switch (pick_nt(np)) {
     case 1: QM_SPLIT_CASE(1); break;
     case 2: QM_SPLIT_CASE(2); break;
@@ -658,9 +680,10 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     QM_CHECK_LAUNCH(ctx);
     ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
     if (nsub3 == 1) {
-      launch_contract_split<T16, TO>(s, ld, hi, lo, pscale, Phi, Plo, np, npts, h3, out, v0);
+      launch_contract_split<T16, TO>(s, ld, hi, lo, pscale, Phi, Plo, np, npts, h3, out, v0, ctx->split_tm);
     } else {
-      launch_contract_split<T16, double>(s, ld, hi, lo, pscale, Phi, Plo, np, npts, h3 / nsub3, rho, (int64_t)0);
+      launch_contract_split<T16, double>(s, ld, hi, lo, pscale, Phi, Plo, np, npts, h3 / nsub3, rho, (int64_t)0,
+                                         ctx->split_tm);
       QM_CHECK_LAUNCH(ctx);
       hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
                          nsub3, out, v0);
@@ -737,6 +760,75 @@ int qm_basis_points(qm_ctx* ctx, qm_mol* mol, void* stream, const double* pts, i
   hipLaunchKernelGGL(k_phi_f64, dim3((n + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0,
                      (hipStream_t)stream, mv, gv, (int64_t)0, n, 1, n, pts, n, mol->nbf, phi);
   QM_CHECK_LAUNCH(ctx);
+  return QM_OK;
+}
+
+int qm_tune_set(qm_ctx* ctx, int knob, int value) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  if (knob == QM_TUNE_SPLIT_ROWS) {
+    QM_REQUIRE(ctx, value == 64 || value == 128, QM_EINVAL, "tune: split rows must be 64 or 128");
+    ctx->split_tm = value;
+    return QM_OK;
+  }
+  return qm_fail(ctx, QM_EINVAL, "tune: unknown knob %d", knob);
+}
+
+int qm_tune_get(qm_ctx* ctx, int knob, int* value) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, value != nullptr, QM_EINVAL, "tune: null output");
+  if (knob == QM_TUNE_SPLIT_ROWS) {
+    *value = ctx->split_tm;
+    return QM_OK;
+  }
+  return qm_fail(ctx, QM_EINVAL, "tune: unknown knob %d", knob);
+}
+
+int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int precision, double* best_ms) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  if ((rc = qm_check_geom(ctx, g))) return rc;
+  QM_REQUIRE(ctx, mol && mol->has_dm, QM_EINVAL, "autotune: molecule without density matrix");
+  hipStream_t s = (hipStream_t)stream;
+  // a slab of whole xy planes, about 1.6M points
+  qm_geom slab = *g;
+  const int64_t plane = (int64_t)g->nx * g->ny;
+  slab.nz = (int)std::max<int64_t>(1, std::min<int64_t>(g->nz, ((int64_t)1 << 21) / std::max<int64_t>(plane, 1)));
+  void* out = nullptr;
+  if ((rc = qm_ws(ctx, WS_MISC, sizeof(float) * (size_t)npoints(&slab), &out))) return rc;
+  std::vector<int> cands;
+  if (precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3) cands = {64, 128};
+  else cands = {ctx->split_tm};
+  hipEvent_t a, b;
+  QM_CHECK_HIP(ctx, hipEventCreate(&a));
+  QM_CHECK_HIP(ctx, hipEventCreate(&b));
+  double best = 1e300;
+  int best_v = ctx->split_tm;
+  for (int v : cands) {
+    ctx->split_tm = v;
+    double t = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {   // first repetition warms caches and workspaces
+      QM_CHECK_HIP(ctx, hipEventRecord(a, s));
+      rc = qm_density(ctx, mol, stream, &slab, 1, precision, QM_F32, out, 0, nullptr);
+      if (rc) break;
+      QM_CHECK_HIP(ctx, hipEventRecord(b, s));
+      QM_CHECK_HIP(ctx, hipEventSynchronize(b));
+      float ms = 0.f;
+      QM_CHECK_HIP(ctx, hipEventElapsedTime(&ms, a, b));
+      if (rep > 0) t = std::min(t, (double)ms);
+    }
+    if (rc) break;
+    if (t < best) {
+      best = t;
+      best_v = v;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  ctx->split_tm = best_v;
+  if (rc) return rc;
+  if (best_ms) *best_ms = best;
   return QM_OK;
 }
 

@@ -81,6 +81,7 @@ struct qm_ctx {
   double* host_scalars = nullptr;   // pinned host buffer for results
   FftPlanCache* fft = nullptr;
   bool prof = false;
+  int split_tm = 128;               // rows per workgroup of the 16-bit split contraction (64 or 128)
   std::vector<ProfRec> recs;
 };
 

@@ -70,9 +70,13 @@ SIGNATURES = {
     "qm_hartree_gto_points": [_vp, _vp, _vp, _vp, _i64, _vp],
     "qm_hartree_gto_grid": [_vp, _vp, _vp, _pg, _i, _vp],
     "qm_vne_classic": [_vp, _vp, _vp, _pg, _i, _i, _vp],
+    "qm_tune_set": [_vp, _i, _i],
+    "qm_tune_get": [_vp, _i, _pi],
+    "qm_autotune_density": [_vp, _vp, _vp, _pg, _i, _pd],
     "qm_profile": [_vp, _i],
     "qm_profile_read": [_vp, _i, _pd, _c.POINTER(_i64)],
 }
+QM_TUNE_SPLIT_ROWS = 0
 QM_K_DENSITY_CONTRACT, QM_K_DENSITY_SPLIT, QM_K_PHI, QM_K_PHI_SPLIT, QM_K_FFT_AXIS, QM_K_GTO, QM_K_JACOBI = range(7)
 _RESTYPE = {"qm_last_error": _c.c_char_p}
 

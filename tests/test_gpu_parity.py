@@ -259,3 +259,60 @@ def test_gto_grid(qm, obasis):
     qm.QMTools.Compute_hartree(vg, mol)
     want = og.potential_grid(sysm, nx, ny, nz, h, o)
     assert _rel(vg.qube, want) <= 1e-10
+
+
+# ---------------------------------------------------------------- contraction tile paths
+@pytest.fixture(scope="module", params=[56, 70])
+def large(request, qm, obasis):
+    """nbf_pad 576 (one column pass of 5 tiles per wave) and 768 (two passes)."""
+    from qmtools import synth
+    basis = qm.BasisSet(BASIS)
+    mol = synth.synthetic_molecule(request.param, 40 + request.param, basis)
+    grid = synth.centred_grid(mol, 16, step_bohr=1.6, pad_ang=2.0)
+    return mol, grid, _system(mol, obasis)
+
+
+def test_density_fp64_large_basis(qm, large):
+    from oracle import density as od
+    mol, grid, sysm = large
+    nx, ny, nz, h, o = grid.geometry()
+    _say("density fp64 large basis", grid.shape, "nbf", mol.nbf)
+    q = qm.QMTools.Compute_density(grid, mol)
+    want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
+    assert _rel(q.qube, want) <= 1e-10
+    assert q.raw_total == pytest.approx(raw, rel=1e-10)
+
+
+@pytest.mark.parametrize("rows", [64, 128])
+def test_density_f16_large_basis_tiles(qm, large, rows):
+    from oracle import density as od
+    from qmtools import autotune
+    mol, grid, sysm = large
+    nx, ny, nz, h, o = grid.geometry()
+    _say("density f16 large basis", grid.shape, "nbf", mol.nbf, "rows", rows)
+    autotune.set("split_rows", rows)
+    try:
+        q = qm.QMTools.Compute_density(grid, mol, precision="f16")
+    finally:
+        autotune.set("split_rows", 128)
+    want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
+    err = _rel(q.qube, want)
+    print(f"f16 rows {rows} nbf {mol.nbf}: normwise error {err:.3e}")
+    assert err <= 1e-6
+    assert abs(q.raw_total - raw) / raw <= 1e-6
+
+
+def test_autotune_cache(qm, small, tmp_path):
+    import json
+    from qmtools import autotune
+    mol, grid, _ = small
+    path = str(tmp_path / "tune.json")
+    _say("autotune", grid.shape)
+    e = autotune.autotune_density(mol, grid, "mixed", path=path)
+    assert e["split_rows"] in (64, 128) and e["ms"] > 0
+    table = json.load(open(path))
+    assert autotune.key_for(mol, "mixed") in table
+    autotune.set("split_rows", 64)
+    again = autotune.autotune_density(mol, grid, "mixed", path=path)   # served from the cache
+    assert again == e and autotune.get("split_rows") == e["split_rows"]
+    autotune.set("split_rows", 128)

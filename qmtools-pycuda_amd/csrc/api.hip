@@ -145,6 +145,8 @@ static void free_mol_device(qm_mol* m) {
   if (m->atoms) (void)hipFree(m->atoms);
   if (m->shells) (void)hipFree(m->shells);
   if (m->prims) (void)hipFree(m->prims);
+  if (m->prims_l2e) (void)hipFree(m->prims_l2e);
+  m->prims_l2e = nullptr;
   if (m->P64) (void)hipFree(m->P64);
   if (m->P16) (void)hipFree(m->P16);
   if (m->Pb16) (void)hipFree(m->Pb16);
@@ -209,11 +211,16 @@ int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr,
   for (int a = 0; a < natoms; ++a)
     at[a] = make_double4(xyz_bohr[3 * a], xyz_bohr[3 * a + 1], xyz_bohr[3 * a + 2], (double)Z[a]);
   std::vector<double2> pr(nprim);
-  for (int p = 0; p < nprim; ++p) pr[p] = make_double2(alpha[p], coeff[p]);
+  std::vector<float2> pr32(nprim);
+  for (int p = 0; p < nprim; ++p) {
+    pr[p] = make_double2(alpha[p], coeff[p]);
+    pr32[p] = make_float2((float)(alpha[p] * 1.4426950408889634074), (float)coeff[p]);
+  }
 
   if (hipMalloc(&m->atoms, sizeof(double4) * natoms) != hipSuccess ||
       hipMalloc(&m->shells, sizeof(DevShell) * nshell) != hipSuccess ||
-      hipMalloc(&m->prims, sizeof(double2) * nprim) != hipSuccess) {
+      hipMalloc(&m->prims, sizeof(double2) * nprim) != hipSuccess ||
+      hipMalloc(&m->prims_l2e, sizeof(float2) * nprim) != hipSuccess) {
     (void)hipGetLastError();
     free_mol_device(m);
     delete m;
@@ -221,7 +228,8 @@ int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr,
   }
   if (hipMemcpy(m->atoms, at.data(), sizeof(double4) * natoms, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->shells, m->h_shells.data(), sizeof(DevShell) * nshell, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(m->prims, pr.data(), sizeof(double2) * nprim, hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(m->prims, pr.data(), sizeof(double2) * nprim, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->prims_l2e, pr32.data(), sizeof(float2) * nprim, hipMemcpyHostToDevice) != hipSuccess) {
     hipError_t e = hipGetLastError();
     free_mol_device(m);
     delete m;

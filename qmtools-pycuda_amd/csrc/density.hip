@@ -125,84 +125,49 @@ __device__ __forceinline__ _Float16 to16<_Float16>(float x) { return (_Float16)x
 template <>
 __device__ __forceinline__ __bf16 to16<__bf16>(float x) { return (__bf16)x; }
 
-// FP32 basis values, split into 16-bit hi + lo after scaling each point by a
-// power of two that brings its largest |phi| to [1, 2): exact scaling, and it
-// keeps small-valued points (far from all nuclei) out of the FP16 subnormals.
-// Pass 1 finds the largest |phi|, pass 2 recomputes and writes.
-template <typename T16>
-__global__ __launch_bounds__(kPhiThreads) void k_phi_split(MolView m, GeomView g, int64_t v0, int nvox, int nsub,
-                                                           int npts, int ld, int nbf_pad, T16* __restrict__ hi,
-                                                           T16* __restrict__ lo, float* __restrict__ pscale) {
-  __shared__ T16 stage_hi[kPhiThreads][8];
-  __shared__ T16 stage_lo[kPhiThreads][8];
+// FP32 basis values Phi^T[f * ld + t] (same layout as FP64), single pass, and
+// the per-point power of two pscale[t] that brings the point's largest |phi|
+// to [1, 2): the split contraction multiplies by it before rounding to 16 bits,
+// which keeps points far from every nucleus out of the FP16 subnormal range.
+// exp(-a r^2) is evaluated as exp2(-(a log2 e) r^2) with a log2 e rounded once
+// on the host (prim_l2e), so the argument carries a single FP32 rounding.
+__global__ __launch_bounds__(kPhiThreads) void k_phi_f32(MolView m, const float2* __restrict__ prim_l2e,
+                                                         GeomView g, int64_t v0, int nvox, int nsub, int npts,
+                                                         int ld, int nbf_pad, float* __restrict__ phi,
+                                                         float* __restrict__ pscale) {
   const int t = blockIdx.x * kPhiThreads + threadIdx.x;
   if (t >= ld) return;
-  const bool live = t < npts;
-  double x = 0.0, y = 0.0, z = 0.0;
-  if (live) chunk_point(g, v0, nvox, nsub, t, x, y, z);
+  if (t >= npts) {
+    for (int f = 0; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.f;
+    pscale[t] = 1.f;
+    return;
+  }
+  double x, y, z;
+  chunk_point(g, v0, nvox, nsub, t, x, y, z);
   // offsets to the nucleus are formed in FP64 and rounded once: rounding x and R
   // separately to FP32 costs ~1e-6 bohr, i.e. ~1e-4 relative error in tight primitives
   float vmax = 0.f;
-  if (live) {
-    for (int s = 0; s < m.nshell; ++s) {
-      const DevShell sh = m.shells[s];
-      const double4 a = m.atoms[sh.atom];
-      const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      float radial = 0.f;
-      for (int p = 0; p < sh.nprim; ++p) {
-        const double2 pc = m.prims[sh.poff + p];
-        radial += (float)pc.y * expf(-(float)pc.x * r2);
-      }
-      float v[5];
-      const int nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
-      for (int c = 0; c < nc; ++c) vmax = fmaxf(vmax, fabsf(v[c]));
-    }
-  }
-  int e = 0;
-  if (vmax > 0.f) frexpf(vmax, &e);          // vmax = f 2^e, f in [0.5, 1)
-  const int sh_e = 1 - e;                    // scaled max in [1, 2)
-  if (pscale) pscale[t] = live ? ldexpf(1.0f, -2 * sh_e) : 0.f;
-  for (int s = 0, f = 0; s < m.nshell; ++s) {
+  for (int s = 0; s < m.nshell; ++s) {
     const DevShell sh = m.shells[s];
-    float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    int nc = 2 * sh.L + 1;
-    if (live) {
-      const double4 a = m.atoms[sh.atom];
-      const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      float radial = 0.f;
-      for (int p = 0; p < sh.nprim; ++p) {
-        const double2 pc = m.prims[sh.poff + p];
-        radial += (float)pc.y * expf(-(float)pc.x * r2);
-      }
-      nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
+    const double4 a = m.atoms[sh.atom];
+    const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float radial = 0.f;
+    for (int p = 0; p < sh.nprim; ++p) {
+      const float2 pc = prim_l2e[sh.poff + p];
+      radial += pc.y * __builtin_amdgcn_exp2f(-pc.x * r2);
     }
-    for (int c = 0; c < nc; ++c, ++f) {
-      const float sv = ldexpf(v[c], sh_e);
-      const T16 h = to16<T16>(sv);
-      const T16 l = to16<T16>(sv - (float)h);
-      stage_hi[threadIdx.x][f & 7] = h;
-      stage_lo[threadIdx.x][f & 7] = l;
-      if ((f & 7) == 7) {
-        const int64_t o = ((int64_t)(f >> 3) * ld + t) * 8;
-        *(uint4*)(hi + o) = *(const uint4*)stage_hi[threadIdx.x];
-        *(uint4*)(lo + o) = *(const uint4*)stage_lo[threadIdx.x];
-      }
-    }
-    if (s == m.nshell - 1) {
-      // pad the last group and the remaining rows with zeros
-      for (; f < nbf_pad; ++f) {
-        stage_hi[threadIdx.x][f & 7] = to16<T16>(0.f);
-        stage_lo[threadIdx.x][f & 7] = to16<T16>(0.f);
-        if ((f & 7) == 7) {
-          const int64_t o = ((int64_t)(f >> 3) * ld + t) * 8;
-          *(uint4*)(hi + o) = *(const uint4*)stage_hi[threadIdx.x];
-          *(uint4*)(lo + o) = *(const uint4*)stage_lo[threadIdx.x];
-        }
-      }
+    float v[5];
+    const int nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
+    for (int c = 0; c < nc; ++c) {
+      phi[(int64_t)(sh.bf + c) * ld + t] = v[c];
+      vmax = fmaxf(vmax, fabsf(v[c]));
     }
   }
+  for (int f = m.nbf; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.f;
+  int e = 0;
+  if (vmax > 0.f) frexpf(vmax, &e);   // vmax = f 2^e, f in [0.5, 1)
+  pscale[t] = ldexpf(1.0f, 1 - e);
 }
 
 // sum over the 16 lanes that share lane/16 (the n index of the MFMA D layout)
@@ -365,12 +330,15 @@ struct Vec8<__bf16> {
   }
 };
 
-// Split precision: X = Ahi Bhi + Ahi Blo + Alo Bhi (FP32 accumulate),
-// rho_t = pscale_t * scale * sum_n (Ahi + Alo)[t][n] X[t][n] (row-dot in FP64).
+// Split precision: A = pscale_t Phi[t][k] is rounded to 16-bit hi + lo while it is
+// staged into LDS (once per workgroup, shared by the 8 waves), then
+// X = Ahi Bhi + Ahi Blo + Alo Bhi (FP32 accumulate), and
+// rho_t = scale / pscale_t * sum_n Phi[t][n] X[t][n] with the FP32 Phi (row-dot in FP64).
 // MFMA 16x16x32: A lane l = (t: l%16, k: 8(l/16)+0..7), B likewise, D lane l, reg r = (t: 4(l/16)+r, n: l%16).
+// Staging: thread (row, k-group) loads 8 values of one row (each load instruction covers 64
+// consecutive rows, 256 B) and writes one 16-byte hi and lo word to LDS.
 template <typename T16, int NT, int TM, typename TO>
-__global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ phi_hi,
-                                                        const T16* __restrict__ phi_lo, int ld,
+__global__ __launch_bounds__(kWG) void k_contract_split(const float* __restrict__ phi, int ld,
                                                         const float* __restrict__ pscale,
                                                         const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
                                                         int nbf_pad, int npts, double scale, TO* __restrict__ out,
@@ -378,7 +346,6 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
   typedef typename Vec8<T16>::type V8;
   constexpr int KG = 4;             // 4 groups of 8 k = 32 k per stage
   constexpr int RT = TM / 16;       // row tiles
-  constexpr int SPT = KG * TM / kWG; // staged uint4 per plane per thread (1 for TM 128)
   __shared__ uint4 Ah[2][KG][TM], Al[2][KG][TM];
   __shared__ double red[kWaves][TM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -386,10 +353,22 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
   const int row0 = blockIdx.x * TM;
   for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
   const int nks = nbf_pad / 32;
-  // staging: KG * TM uint4 per plane; threads below KG * TM load one each
   const bool stager = threadIdx.x < KG * TM;
-  const int sg = threadIdx.x / TM, st = threadIdx.x % TM;
-  (void)SPT;
+  const int sr = threadIdx.x % TM, sg = threadIdx.x / TM;     // staged row and k-group
+  const float sc = stager ? pscale[row0 + sr] : 1.f;
+  const float* src = phi + (int64_t)(sg * 8) * ld + row0 + sr;
+  auto stage_store = [&](int buf, const float* v) {
+    V8 h, l;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float x = v[q] * sc;
+      const T16 hq = (T16)x;
+      h[q] = hq;
+      l[q] = (T16)(x - (float)hq);
+    }
+    Ah[buf][sg][sr] = *(const uint4*)&h;
+    Al[buf][sg][sr] = *(const uint4*)&l;
+  };
   for (int c0 = 0; c0 < nbf_pad; c0 += kWaves * NT * 16) {
     const int wc0 = c0 + wave * NT * 16;
     f4 acc[RT][NT];
@@ -397,20 +376,19 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0, 0, 0, 0};
-    uint4 ph{}, pl{};
+    float pre[8];
     if (stager) {
-      ph = *(const uint4*)(phi_hi + ((int64_t)sg * ld + row0 + st) * 8);
-      pl = *(const uint4*)(phi_lo + ((int64_t)sg * ld + row0 + st) * 8);
-      Ah[0][sg][st] = ph;
-      Al[0][sg][st] = pl;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) pre[q] = src[(int64_t)q * ld];
+      stage_store(0, pre);
     }
     __syncthreads();
     for (int s = 0; s < nks; ++s) {
       const bool more = s + 1 < nks;
       if (more && stager) {
-        const int64_t o = ((int64_t)((s + 1) * KG + sg) * ld + row0 + st) * 8;
-        ph = *(const uint4*)(phi_hi + o);
-        pl = *(const uint4*)(phi_lo + o);
+        const float* nsrc = src + (int64_t)((s + 1) * 32) * ld;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pre[q] = nsrc[(int64_t)q * ld];
       }
       if (wc0 < nbf_pad) {
         V8 bh[NT], bl[NT];
@@ -440,10 +418,7 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
           }
         }
       }
-      if (more && stager) {
-        Ah[(s + 1) & 1][sg][st] = ph;
-        Al[(s + 1) & 1][sg][st] = pl;
-      }
+      if (more && stager) stage_store((s + 1) & 1, pre);
       __syncthreads();
     }
     if (wc0 < nbf_pad) {
@@ -454,12 +429,11 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
         for (int ct = 0; ct < NT; ++ct) {
           const int n = wc0 + ct * 16 + li;
           if (n < nbf_pad) {
-            const int64_t base = ((int64_t)(n >> 3) * ld + row0 + rt * 16 + 4 * lk) * 8 + (n & 7);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const float v = (float)phi_hi[base + 8 * r] + (float)phi_lo[base + 8 * r];
-              part[r] += (double)(acc[rt][ct][r] * v);
-            }
+            const float4 v = *(const float4*)(phi + (int64_t)n * ld + row0 + rt * 16 + 4 * lk);
+            part[0] += (double)(acc[rt][ct][0] * v.x);
+            part[1] += (double)(acc[rt][ct][1] * v.y);
+            part[2] += (double)(acc[rt][ct][2] * v.z);
+            part[3] += (double)(acc[rt][ct][3] * v.w);
           }
         }
 #pragma unroll
@@ -476,7 +450,7 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const T16* __restrict__ 
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) v += red[w][threadIdx.x];
     const int t = row0 + threadIdx.x;
-    if (t < npts) store_rho<TO>(out, v0 + t, v * (double)pscale[t] * scale);
+    if (t < npts) store_rho<TO>(out, v0 + t, v / (double)pscale[t] * scale);
   }
 }
 
@@ -572,16 +546,16 @@ void launch_contract_f64(hipStream_t s, int ld, const double* phi, const double*
 }
 
 template <typename T16, typename TO>
-void launch_contract_split(hipStream_t s, int ld, const T16* hi, const T16* lo, const float* pscale, const T16* Ph,
+void launch_contract_split(hipStream_t s, int ld, const float* phi, const float* pscale, const T16* Ph,
                            const T16* Pl, int np, int npts, double scale, TO* out, int64_t v0, int tm) {
   const dim3 grid(ld / tm), block(kWG);
 #define QM_SPLIT_CASE(NT)                                                                                        \
   if (tm == 64)                                                                                                  \
-    hipLaunchKernelGGL((k_contract_split<T16, NT, 64, TO>), grid, block, 0, s, hi, lo, ld, pscale, Ph, Pl, np, npts, \
+    hipLaunchKernelGGL((k_contract_split<T16, NT, 64, TO>), grid, block, 0, s, phi, ld, pscale, Ph, Pl, np, npts,  \
                        scale, out, v0);                                                                          \
   else                                                                                                           \
-    hipLaunchKernelGGL((k_contract_split<T16, NT, 128, TO>), grid, block, 0, s, hi, lo, ld, pscale, Ph, Pl, np,     \
-                       npts, scale, out, v0)
+    hipLaunchKernelGGL((k_contract_split<T16, NT, 128, TO>), grid, block, 0, s, phi, ld, pscale, Ph, Pl, np, npts, \
+                       scale, out, v0)
   switch (pick_nt(np)) {
     case 1: QM_SPLIT_CASE(1); break;
     case 2: QM_SPLIT_CASE(2); break;
@@ -644,14 +618,14 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
   const int np = mol->nbf_pad;
   const int nsub3 = nsub * nsub * nsub;
   const int64_t nvox_tot = npoints(g);
-  int64_t maxpts = (int64_t)(kPhiBudget / ((size_t)np * 2 * sizeof(uint16_t)));
+  int64_t maxpts = (int64_t)(kPhiBudget / ((size_t)np * sizeof(float)));
   maxpts = std::max<int64_t>(maxpts / kTileM * kTileM, kTileM);
   int64_t nvox = std::max<int64_t>(maxpts / nsub3, 1);
   nvox = std::min<int64_t>(nvox, nvox_tot);
   nvox = std::min<int64_t>(nvox, ((int64_t)1 << 30) / nsub3);
   const int64_t ldmax = round_up(nvox * nsub3, kTileM);
   void* phi = nullptr;
-  int rc = qm_ws(ctx, WS_PHI, sizeof(uint16_t) * 2 * (size_t)np * ldmax, &phi);
+  int rc = qm_ws(ctx, WS_PHI, sizeof(float) * (size_t)np * ldmax, &phi);
   if (rc) return rc;
   float* pscale = nullptr;
   rc = qm_ws(ctx, WS_PHI2, sizeof(float) * ldmax, (void**)&pscale);
@@ -670,20 +644,19 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     const int nv = (int)std::min<int64_t>(nvox, nvox_tot - v0);
     const int npts = nv * nsub3;
     const int ld = (int)round_up(npts, kTileM);
-    T16* hi = (T16*)phi;
-    T16* lo = hi + (int64_t)np * ld;
     {
       ProfScope ps(ctx, s, QM_K_PHI_SPLIT);
-      hipLaunchKernelGGL(k_phi_split<T16>, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s,
-                         mv, gv, v0, nv, nsub, npts, ld, np, hi, lo, pscale);
+      hipLaunchKernelGGL(k_phi_f32, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s, mv,
+                         (const float2*)mol->prims_l2e, gv, v0, nv, nsub, npts, ld, np, (float*)phi, pscale);
     }
     QM_CHECK_LAUNCH(ctx);
     ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
     if (nsub3 == 1) {
-      launch_contract_split<T16, TO>(s, ld, hi, lo, pscale, Phi, Plo, np, npts, h3, out, v0, ctx->split_tm);
+      launch_contract_split<T16, TO>(s, ld, (const float*)phi, pscale, Phi, Plo, np, npts, h3, out, v0,
+                                     ctx->split_tm);
     } else {
-      launch_contract_split<T16, double>(s, ld, hi, lo, pscale, Phi, Plo, np, npts, h3 / nsub3, rho, (int64_t)0,
-                                         ctx->split_tm);
+      launch_contract_split<T16, double>(s, ld, (const float*)phi, pscale, Phi, Plo, np, npts, h3 / nsub3, rho,
+                                         (int64_t)0, ctx->split_tm);
       QM_CHECK_LAUNCH(ctx);
       hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
                          nsub3, out, v0);

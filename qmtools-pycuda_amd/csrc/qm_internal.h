@@ -90,6 +90,7 @@ struct qm_mol {
   double4* atoms = nullptr;
   DevShell* shells = nullptr;
   double2* prims = nullptr;
+  float2* prims_l2e = nullptr;      // (alpha log2 e, coeff) in FP32 for the FP32 basis kernel
   // host copies (used to build per-request tables)
   std::vector<double> xyz, Z;
   std::vector<DevShell> h_shells;

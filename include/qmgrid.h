@@ -133,7 +133,9 @@ int qm_scale(qm_ctx* ctx, void* stream, int dtype, void* a, int64_t n, double fa
 
 /* Nuclear seed: writes Z * prod_d |a_d - i_d h|/h to the 8 nodes around each
  * nucleus (node positions origin + i h); other points are left untouched.
- * Nodes outside the grid are skipped.  Replaces gpu_density_seed
+ * Nodes outside the grid are skipped.  A node shared by several nuclei gets
+ * the value of the nucleus with the highest index (the reference lets such
+ * writes race).  Replaces gpu_density_seed
  * (kernel_utils.cu:5-50) and Compute_qseed (qmtools.py:2645-2672). */
 int qm_seed(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int dtype, void* out);
 
@@ -201,6 +203,32 @@ int qm_hartree_gto_grid(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
  * Replaces gpu_vne_classic (kernel_vne_classic.cu:13-84), Compute_VNe_classic (qmtools.py:2750-2790). */
 int qm_vne_classic(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int ndiv, int dtype,
                    void* out);
+
+/* ---------------------------------------------------------------- batches */
+
+/* Density of nmol molecules on their own grids (sizes and steps may differ),
+ * subgrid 1, in fused launches: the grids are cut into chunks packed into
+ * rounds of the basis-value workspace; each round is one basis-value launch
+ * plus one contraction launch per column-tile class.  outs[i] receives
+ * molecule i's density (out_dtype), normalised to its sum Z if `normalize`.
+ * The per-point arithmetic is the single-molecule path's (qm_density).
+ * No reference counterpart (the reference runs one molecule per call). */
+int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* const* mols, const qm_geom* geoms,
+                     void* const* outs, int precision, int out_dtype, int normalize);
+
+/* nsweeps Jacobi sweeps on n grids at once (each its own size/step), one
+ * launch per sweep for the whole batch.  With use_graph, each block of
+ * `block` sweeps (even) is captured once into a hipGraph and replayed.
+ * precision QM_PREC_FP64: rho[i], v[i], tmp[i] FP64; the outer layer of v[i]
+ *   is held (zero or multipole values).
+ * precision QM_PREC_FP32: rho[i], v[i] FP64, tmp unused; each block forms the
+ *   FP64 residual and runs its sweeps in FP32 on the correction (as qm_jacobi).
+ * Waits for the caller's stream before enqueueing (the item table is
+ * uploaded from host memory), then returns with the work enqueued and the
+ * caller's stream ordered after it.  No tolerance checks (fixed sweep
+ * count).  No reference counterpart. */
+int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* geoms, const void* const* rho,
+                    void* const* v, void* const* tmp, int precision, int64_t nsweeps, int block, int use_graph);
 
 /* ---------------------------------------------------------------- tuning */
 

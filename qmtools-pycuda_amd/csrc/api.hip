@@ -95,8 +95,10 @@ int qm_ctx_destroy(qm_ctx* ctx) {
   for (auto& w : ctx->ws)
     if (w.ptr) (void)hipFree(w.ptr);
   if (ctx->host_scalars) (void)hipHostFree(ctx->host_scalars);
+  qm_reclaim_graphs(ctx, true);
   qm_fft_cache_free(ctx->fft);
   clear_records(ctx);
+  if (ctx->graph_stream) (void)hipStreamDestroy(ctx->graph_stream);
   delete ctx;
   return QM_OK;
 }

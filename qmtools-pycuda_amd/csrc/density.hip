@@ -84,10 +84,10 @@ __device__ __forceinline__ void chunk_point(const GeomView& g, int64_t v0, int n
 
 // exp(-a r2) == 0 in double for a r2 > 745.2: a shell whose smallest exponent
 // passes that bound contributes exact zeros, so skipping it changes nothing.
-__global__ __launch_bounds__(kPhiThreads) void k_phi_f64(MolView m, GeomView g, int64_t v0, int nvox, int nsub,
-                                                         int npts, const double* __restrict__ pts, int ld,
-                                                         int nbf_pad, double* __restrict__ phi) {
-  const int t = blockIdx.x * kPhiThreads + threadIdx.x;
+__device__ __forceinline__ void phi_f64_body(int blk, const MolView& m, const GeomView& g, int64_t v0, int nvox,
+                                             int nsub, int npts, const double* __restrict__ pts, int ld,
+                                             int nbf_pad, double* __restrict__ phi) {
+  const int t = blk * kPhiThreads + threadIdx.x;
   if (t >= ld) return;
   if (t >= npts) {
     for (int f = 0; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.0;
@@ -118,6 +118,12 @@ __global__ __launch_bounds__(kPhiThreads) void k_phi_f64(MolView m, GeomView g, 
   for (int f = m.nbf; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.0;
 }
 
+__global__ __launch_bounds__(kPhiThreads) void k_phi_f64(MolView m, GeomView g, int64_t v0, int nvox, int nsub,
+                                                         int npts, const double* __restrict__ pts, int ld,
+                                                         int nbf_pad, double* __restrict__ phi) {
+  phi_f64_body(blockIdx.x, m, g, v0, nvox, nsub, npts, pts, ld, nbf_pad, phi);
+}
+
 template <typename T16>
 __device__ __forceinline__ T16 to16(float x);
 template <>
@@ -131,11 +137,10 @@ __device__ __forceinline__ __bf16 to16<__bf16>(float x) { return (__bf16)x; }
 // which keeps points far from every nucleus out of the FP16 subnormal range.
 // exp(-a r^2) is evaluated as exp2(-(a log2 e) r^2) with a log2 e rounded once
 // on the host (prim_l2e), so the argument carries a single FP32 rounding.
-__global__ __launch_bounds__(kPhiThreads) void k_phi_f32(MolView m, const float2* __restrict__ prim_l2e,
-                                                         GeomView g, int64_t v0, int nvox, int nsub, int npts,
-                                                         int ld, int nbf_pad, float* __restrict__ phi,
-                                                         float* __restrict__ pscale) {
-  const int t = blockIdx.x * kPhiThreads + threadIdx.x;
+__device__ __forceinline__ void phi_f32_body(int blk, const MolView& m, const float2* __restrict__ prim_l2e,
+                                             const GeomView& g, int64_t v0, int nvox, int nsub, int npts, int ld,
+                                             int nbf_pad, float* __restrict__ phi, float* __restrict__ pscale) {
+  const int t = blk * kPhiThreads + threadIdx.x;
   if (t >= ld) return;
   if (t >= npts) {
     for (int f = 0; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.f;
@@ -168,6 +173,13 @@ __global__ __launch_bounds__(kPhiThreads) void k_phi_f32(MolView m, const float2
   int e = 0;
   if (vmax > 0.f) frexpf(vmax, &e);   // vmax = f 2^e, f in [0.5, 1)
   pscale[t] = ldexpf(1.0f, 1 - e);
+}
+
+__global__ __launch_bounds__(kPhiThreads) void k_phi_f32(MolView m, const float2* __restrict__ prim_l2e,
+                                                         GeomView g, int64_t v0, int nvox, int nsub, int npts,
+                                                         int ld, int nbf_pad, float* __restrict__ phi,
+                                                         float* __restrict__ pscale) {
+  phi_f32_body(blockIdx.x, m, prim_l2e, g, v0, nvox, nsub, npts, ld, nbf_pad, phi, pscale);
 }
 
 // sum over the 16 lanes that share lane/16 (the n index of the MFMA D layout)
@@ -203,15 +215,15 @@ constexpr int kWG = kWaves * 64;
 // FP64: MFMA f64 16x16x4.  A lane l = (t: l%16, k: l/16), B lane l = (k: l/16, n: l%16),
 // D lane l, register r = (t: 4r + l/16, n: l%16).
 template <int NT, typename TO>
-__global__ __launch_bounds__(kWG) void k_contract_f64(const double* __restrict__ phi, int ld,
-                                                      const double* __restrict__ P, int nbf_pad, int npts,
-                                                      double scale, TO* __restrict__ out, int64_t v0) {
+__device__ __forceinline__ void contract_f64_body(int blk, const double* __restrict__ phi, int ld,
+                                                  const double* __restrict__ P, int nbf_pad, int npts,
+                                                  double scale, TO* __restrict__ out, int64_t v0) {
   constexpr int TM = 64, KS = 16, PAD = 16;
   __shared__ double As[2][KS][TM + PAD];
   __shared__ double red[kWaves][TM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int row0 = blockIdx.x * TM;
+  const int row0 = blk * TM;
   for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
   const int nks = nbf_pad / KS;
   for (int c0 = 0; c0 < nbf_pad; c0 += kWaves * NT * 16) {
@@ -313,6 +325,13 @@ __global__ __launch_bounds__(kWG) void k_contract_f64(const double* __restrict__
   }
 }
 
+template <int NT, typename TO>
+__global__ __launch_bounds__(kWG) void k_contract_f64(const double* __restrict__ phi, int ld,
+                                                      const double* __restrict__ P, int nbf_pad, int npts,
+                                                      double scale, TO* __restrict__ out, int64_t v0) {
+  contract_f64_body<NT, TO>(blockIdx.x, phi, ld, P, nbf_pad, npts, scale, out, v0);
+}
+
 template <typename T16>
 struct Vec8;
 template <>
@@ -338,11 +357,11 @@ struct Vec8<__bf16> {
 // Staging: thread (row, k-group) loads 8 values of one row (each load instruction covers 64
 // consecutive rows, 256 B) and writes one 16-byte hi and lo word to LDS.
 template <typename T16, int NT, int TM, typename TO>
-__global__ __launch_bounds__(kWG) void k_contract_split(const float* __restrict__ phi, int ld,
-                                                        const float* __restrict__ pscale,
-                                                        const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
-                                                        int nbf_pad, int npts, double scale, TO* __restrict__ out,
-                                                        int64_t v0) {
+__device__ __forceinline__ void contract_split_body(int blk, const float* __restrict__ phi, int ld,
+                                                    const float* __restrict__ pscale,
+                                                    const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
+                                                    int nbf_pad, int npts, double scale, TO* __restrict__ out,
+                                                    int64_t v0) {
   typedef typename Vec8<T16>::type V8;
   constexpr int KG = 4;             // 4 groups of 8 k = 32 k per stage
   constexpr int RT = TM / 16;       // row tiles
@@ -350,7 +369,7 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const float* __restrict_
   __shared__ double red[kWaves][TM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int row0 = blockIdx.x * TM;
+  const int row0 = blk * TM;
   for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
   const int nks = nbf_pad / 32;
   const bool stager = threadIdx.x < KG * TM;
@@ -452,6 +471,15 @@ __global__ __launch_bounds__(kWG) void k_contract_split(const float* __restrict_
     const int t = row0 + threadIdx.x;
     if (t < npts) store_rho<TO>(out, v0 + t, v / (double)pscale[t] * scale);
   }
+}
+
+template <typename T16, int NT, int TM, typename TO>
+__global__ __launch_bounds__(kWG) void k_contract_split(const float* __restrict__ phi, int ld,
+                                                        const float* __restrict__ pscale,
+                                                        const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
+                                                        int nbf_pad, int npts, double scale, TO* __restrict__ out,
+                                                        int64_t v0) {
+  contract_split_body<T16, NT, TM, TO>(blockIdx.x, phi, ld, pscale, p_hi, p_lo, nbf_pad, npts, scale, out, v0);
 }
 
 // Sum the nsub^3 sub-point densities of each voxel: out[v0 + v] = sum_s rho[s * nvox + v].
@@ -806,3 +834,271 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched density: molecules of uneven size in fused launches.
+//
+// Every molecule's grid is cut into chunks of at most kBatchChunk points; the
+// chunks are packed into rounds whose basis values fit one Phi workspace.  A
+// round is one basis-value launch over all its chunks plus one contraction
+// launch per column-tile class NT (chunks sorted by NT, so waves are not left
+// idle by small molecules).  A workgroup finds its chunk by binary search on
+// the chunk table's first-block numbers and runs the same per-tile code as the
+// single-molecule kernels, so a molecule's density is identical alone and in
+// a batch.
+namespace {
+
+constexpr int kBatchChunk = 1 << 20;
+
+struct BChunk {
+  MolView mol;
+  const float2* l2e;
+  GeomView g;
+  int64_t v0;
+  int nvox;
+  int ld;
+  int nbf_pad;
+  int nt;
+  void* phi;
+  float* pscale;
+  const void* ph;       // P64 (FP64) or 16-bit hi plane
+  const void* pl;       // 16-bit lo plane
+  void* out;
+  double scale;
+  int blk0_phi;         // first block in the round's basis-value launch
+  int blk0;             // first block in its contraction launch
+};
+
+__device__ __forceinline__ int find_chunk(const BChunk* __restrict__ c, int n, int b, bool phi_launch) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int first = phi_launch ? c[mid].blk0_phi : c[mid].blk0;
+    if (first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kPhiThreads) void kb_phi_f64(const BChunk* __restrict__ c, int n) {
+  const int i = find_chunk(c, n, blockIdx.x, true);
+  const BChunk& q = c[i];
+  phi_f64_body(blockIdx.x - q.blk0_phi, q.mol, q.g, q.v0, q.nvox, 1, q.nvox, nullptr, q.ld, q.nbf_pad,
+               (double*)q.phi);
+}
+
+__global__ __launch_bounds__(kPhiThreads) void kb_phi_f32(const BChunk* __restrict__ c, int n) {
+  const int i = find_chunk(c, n, blockIdx.x, true);
+  const BChunk& q = c[i];
+  phi_f32_body(blockIdx.x - q.blk0_phi, q.mol, q.l2e, q.g, q.v0, q.nvox, 1, q.nvox, q.ld, q.nbf_pad, (float*)q.phi,
+               q.pscale);
+}
+
+template <int NT, typename TO>
+__global__ __launch_bounds__(kWG) void kb_contract_f64(const BChunk* __restrict__ c, int n) {
+  const int i = find_chunk(c, n, blockIdx.x, false);
+  const BChunk& q = c[i];
+  contract_f64_body<NT, TO>(blockIdx.x - q.blk0, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad,
+                            q.nvox, q.scale, (TO*)q.out, q.v0);
+}
+
+template <typename T16, int NT, int TM, typename TO>
+__global__ __launch_bounds__(kWG) void kb_contract_split(const BChunk* __restrict__ c, int n) {
+  const int i = find_chunk(c, n, blockIdx.x, false);
+  const BChunk& q = c[i];
+  contract_split_body<T16, NT, TM, TO>(blockIdx.x - q.blk0, (const float*)q.phi, q.ld, q.pscale,
+                                       (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad, q.nvox, q.scale,
+                                       (TO*)q.out, q.v0);
+}
+
+template <typename TO>
+void launch_batch_f64(hipStream_t s, int nt, const BChunk* c, int n, int blocks) {
+  switch (nt) {
+    case 1: hipLaunchKernelGGL((kb_contract_f64<1, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); break;
+    case 2: hipLaunchKernelGGL((kb_contract_f64<2, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); break;
+    case 3: hipLaunchKernelGGL((kb_contract_f64<3, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); break;
+    case 4: hipLaunchKernelGGL((kb_contract_f64<4, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); break;
+    default: hipLaunchKernelGGL((kb_contract_f64<5, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); break;
+  }
+}
+
+template <typename T16, typename TO>
+void launch_batch_split(hipStream_t s, int nt, int tm, const BChunk* c, int n, int blocks) {
+#define QM_BSPLIT(NT)                                                                                    \
+  if (tm == 64)                                                                                          \
+    hipLaunchKernelGGL((kb_contract_split<T16, NT, 64, TO>), dim3(blocks), dim3(kWG), 0, s, c, n);      \
+  else                                                                                                   \
+    hipLaunchKernelGGL((kb_contract_split<T16, NT, 128, TO>), dim3(blocks), dim3(kWG), 0, s, c, n)
+  switch (nt) {
+    case 1: QM_BSPLIT(1); break;
+    case 2: QM_BSPLIT(2); break;
+    case 3: QM_BSPLIT(3); break;
+    case 4: QM_BSPLIT(4); break;
+    default: QM_BSPLIT(5); break;
+  }
+#undef QM_BSPLIT
+}
+
+struct Round {
+  size_t first, count;                      // chunk range in the table (sorted by NT within the round)
+  std::vector<std::pair<int, std::pair<size_t, size_t>>> groups;   // (nt, [first, count))
+  std::vector<int> group_blocks;
+  int phi_blocks;
+};
+
+}  // namespace
+
+extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* const* mols, const qm_geom* geoms,
+                                void* const* outs, int precision, int out_dtype, int normalize) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, nmol >= 0 && (nmol == 0 || (mols && geoms && outs)), QM_EINVAL, "density_batch: bad arguments");
+  QM_REQUIRE(ctx, precision == QM_PREC_FP64 || precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3,
+             QM_EINVAL, "density_batch: bad precision");
+  QM_REQUIRE(ctx, out_dtype == QM_F64 || out_dtype == QM_F32, QM_EINVAL, "density_batch: bad output dtype");
+  if (nmol == 0) return QM_OK;
+  for (int i = 0; i < nmol; ++i) {
+    QM_REQUIRE(ctx, mols[i] && mols[i]->has_dm && outs[i], QM_EINVAL, "density_batch: item %d incomplete", i);
+    if ((rc = qm_check_geom(ctx, &geoms[i]))) return rc;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const bool f64 = precision == QM_PREC_FP64;
+  const size_t elt = f64 ? sizeof(double) : sizeof(float);
+  const int tm = f64 ? 64 : ctx->split_tm;
+  // chunks per molecule, packed into rounds of at most kPhiBudget bytes of basis values
+  std::vector<BChunk> table;
+  std::vector<Round> rounds;
+  std::vector<BChunk> cur;
+  size_t cur_bytes = 0, cur_ps = 0;
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    Round r;
+    r.first = table.size();
+    r.count = cur.size();
+    std::stable_sort(cur.begin(), cur.end(), [](const BChunk& a, const BChunk& b) { return a.nt < b.nt; });
+    int pb = 0;
+    for (auto& c : cur) {
+      c.blk0_phi = pb;
+      pb += (c.ld + kPhiThreads - 1) / kPhiThreads;
+    }
+    r.phi_blocks = pb;
+    size_t k = 0;
+    while (k < cur.size()) {
+      size_t e = k;
+      int blocks = 0;
+      while (e < cur.size() && cur[e].nt == cur[k].nt) {
+        cur[e].blk0 = blocks;
+        blocks += cur[e].ld / tm;
+        ++e;
+      }
+      r.groups.push_back({cur[k].nt, {r.first + k, e - k}});
+      r.group_blocks.push_back(blocks);
+      k = e;
+    }
+    table.insert(table.end(), cur.begin(), cur.end());
+    rounds.push_back(r);
+    cur.clear();
+    cur_bytes = 0;
+    cur_ps = 0;
+  };
+  for (int i = 0; i < nmol; ++i) {
+    qm_mol* m = mols[i];
+    const int np = m->nbf_pad;
+    const int64_t N = npoints(&geoms[i]);
+    const int64_t cap = std::max<int64_t>(kTileM, std::min<int64_t>(kBatchChunk,
+                                          (int64_t)(kPhiBudget / ((size_t)np * elt)) / kTileM * kTileM));
+    for (int64_t v0 = 0; v0 < N; v0 += cap) {
+      BChunk c{};
+      c.mol = m->view();
+      c.l2e = m->prims_l2e;
+      c.g = geom_view(&geoms[i]);
+      c.v0 = v0;
+      c.nvox = (int)std::min<int64_t>(cap, N - v0);
+      c.ld = (int)round_up(c.nvox, kTileM);
+      c.nbf_pad = np;
+      c.nt = pick_nt(np);
+      const size_t bytes = (size_t)np * c.ld * elt;
+      if (cur_bytes + bytes > kPhiBudget) flush();
+      c.phi = (void*)(uintptr_t)cur_bytes;            // offsets for now, rebased below
+      c.pscale = (float*)(uintptr_t)cur_ps;
+      cur_bytes += bytes;
+      cur_ps += (size_t)c.ld * sizeof(float);
+      const double h3 = geoms[i].step * geoms[i].step * geoms[i].step;
+      if (f64) {
+        c.ph = m->P64;
+        c.pl = nullptr;
+        c.scale = h3;
+      } else {
+        const bool half = precision == QM_PREC_F16X3;
+        const uint16_t* base = (const uint16_t*)(half ? m->P16 : m->Pb16);
+        c.ph = base;
+        c.pl = base + (size_t)np * np;
+        c.scale = h3 / m->pfac;
+      }
+      c.out = outs[i];
+      cur.push_back(c);
+    }
+  }
+  flush();
+  // workspaces: basis values of one round, point scales, and the chunk table
+  size_t max_bytes = 0, max_ps = 0;
+  for (const Round& r : rounds) {
+    size_t b = 0, p = 0;
+    for (size_t k = 0; k < r.count; ++k) {
+      const BChunk& c = table[r.first + k];
+      b = std::max(b, (size_t)(uintptr_t)c.phi + (size_t)c.nbf_pad * c.ld * elt);
+      p = std::max(p, (size_t)(uintptr_t)c.pscale + (size_t)c.ld * sizeof(float));
+    }
+    max_bytes = std::max(max_bytes, b);
+    max_ps = std::max(max_ps, p);
+  }
+  void *phi = nullptr, *ps = nullptr, *dtab = nullptr;
+  if ((rc = qm_ws(ctx, WS_PHI, max_bytes, &phi))) return rc;
+  if ((rc = qm_ws(ctx, WS_PHI2, std::max<size_t>(max_ps, 16), &ps))) return rc;
+  if ((rc = qm_ws(ctx, WS_BATCH, sizeof(BChunk) * table.size(), &dtab))) return rc;
+  for (BChunk& c : table) {
+    c.phi = (char*)phi + (size_t)(uintptr_t)c.phi;
+    c.pscale = (float*)((char*)ps + (size_t)(uintptr_t)c.pscale);
+  }
+  QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, table.data(), sizeof(BChunk) * table.size(), hipMemcpyHostToDevice, s));
+  QM_CHECK_HIP(ctx, hipStreamSynchronize(s));    // the host table is pageable and goes out of scope
+  const BChunk* dt = (const BChunk*)dtab;
+  for (const Round& r : rounds) {
+    {
+      ProfScope pp(ctx, s, f64 ? QM_K_PHI : QM_K_PHI_SPLIT);
+      if (f64)
+        hipLaunchKernelGGL(kb_phi_f64, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first, (int)r.count);
+      else
+        hipLaunchKernelGGL(kb_phi_f32, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first, (int)r.count);
+    }
+    QM_CHECK_LAUNCH(ctx);
+    for (size_t gi = 0; gi < r.groups.size(); ++gi) {
+      const int nt = r.groups[gi].first;
+      const BChunk* gc = dt + r.groups[gi].second.first;
+      const int gn = (int)r.groups[gi].second.second;
+      const int blocks = r.group_blocks[gi];
+      ProfScope pc(ctx, s, f64 ? QM_K_DENSITY_CONTRACT : QM_K_DENSITY_SPLIT);
+      if (f64) {
+        if (out_dtype == QM_F64) launch_batch_f64<double>(s, nt, gc, gn, blocks);
+        else launch_batch_f64<float>(s, nt, gc, gn, blocks);
+      } else if (precision == QM_PREC_F16X3) {
+        if (out_dtype == QM_F64) launch_batch_split<_Float16, double>(s, nt, tm, gc, gn, blocks);
+        else launch_batch_split<_Float16, float>(s, nt, tm, gc, gn, blocks);
+      } else {
+        if (out_dtype == QM_F64) launch_batch_split<__bf16, double>(s, nt, tm, gc, gn, blocks);
+        else launch_batch_split<__bf16, float>(s, nt, tm, gc, gn, blocks);
+      }
+      QM_CHECK_LAUNCH(ctx);
+    }
+  }
+  if (normalize) {
+    double* tot = nullptr;
+    if ((rc = qm_ws(ctx, WS_BATCH2, sizeof(double) * (size_t)nmol, (void**)&tot))) return rc;
+    for (int i = 0; i < nmol; ++i) {
+      if ((rc = qm_reduce_dev(ctx, s, QM_RED_SUM, out_dtype, outs[i], nullptr, npoints(&geoms[i]), tot + i))) return rc;
+      if ((rc = qm_scale_dev(ctx, s, out_dtype, outs[i], npoints(&geoms[i]), nullptr, mols[i]->qtot, tot + i)))
+        return rc;
+    }
+  }
+  return QM_OK;
+}

@@ -329,3 +329,248 @@ int qm_jacobi(qm_ctx* ctx, void* stream, const qm_geom* g, int precision, const 
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched Jacobi: many grids per launch, fixed sweep counts, hipGraph replay.
+//
+// Each item is one grid (its own size and step).  A sweep over the whole batch
+// is one launch: workgroups find their item by binary search on the item
+// table's first-block numbers.  A block of sweeps (FP64: 2 K sweeps; FP32:
+// residual, clear, K sweeps, correction) is captured once into a hipGraph on a
+// context-owned stream and replayed; the caller's stream is ordered before and
+// after with events.  Buffers are cleared with a fill kernel, not a memset
+// node.
+namespace {
+
+struct JItem {
+  int nx, ny, nz;
+  double h;
+  const void* rho;    // FP64 rho (FP64 path) / FP32 residual (FP32 path)
+  void* a;            // V (FP64) / e (FP32)
+  void* b;            // tmp (FP64) / t (FP32)
+  const double* rho64;
+  double* v64;
+  float* r32;
+  int blk0, bx, by;
+};
+
+__device__ __forceinline__ int find_item(const JItem* __restrict__ it, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (it[mid].blk0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ bool item_point(const JItem& q, int local, int& i, int& j, int& k) {
+  const int bxi = local % q.bx;
+  const int rest = local / q.bx;
+  const int byi = rest % q.by;
+  k = rest / q.by;
+  i = bxi * kBX + threadIdx.x;
+  j = byi * kBY + threadIdx.y;
+  return i < q.nx && j < q.ny;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBX* kBY) void kb_sweep(const JItem* __restrict__ items, int n, int dir) {
+  const int it = find_item(items, n, blockIdx.x);
+  const JItem& q = items[it];
+  int i, j, k;
+  if (!item_point(q, blockIdx.x - q.blk0, i, j, k)) return;
+  const T* rho = (const T*)q.rho;
+  const T* v = (const T*)(dir ? q.b : q.a);
+  T* vout = (T*)(dir ? q.a : q.b);
+  const int64_t sy = q.nx, sz = (int64_t)q.nx * q.ny;
+  const int64_t idx = i + sy * j + sz * k;
+  if (i == 0 || j == 0 || k == 0 || i == q.nx - 1 || j == q.ny - 1 || k == q.nz - 1) {
+    vout[idx] = v[idx];
+    return;
+  }
+  T s = v[idx - 1] + v[idx + 1];
+  s += v[idx + sy];
+  s += v[idx - sy];
+  s += v[idx + sz];
+  s += v[idx - sz];
+  s -= rho[idx] / (T)q.h;
+  vout[idx] = s / (T)6;
+}
+
+__global__ __launch_bounds__(kBX* kBY) void kb_residual(const JItem* __restrict__ items, int n) {
+  const int it = find_item(items, n, blockIdx.x);
+  const JItem& q = items[it];
+  int i, j, k;
+  if (!item_point(q, blockIdx.x - q.blk0, i, j, k)) return;
+  const int64_t sy = q.nx, sz = (int64_t)q.nx * q.ny;
+  const int64_t idx = i + sy * j + sz * k;
+  float* e = (float*)q.a;
+  float* t = (float*)q.b;
+  e[idx] = 0.f;
+  t[idx] = 0.f;
+  if (i == 0 || j == 0 || k == 0 || i == q.nx - 1 || j == q.ny - 1 || k == q.nz - 1) {
+    q.r32[idx] = 0.f;
+    return;
+  }
+  const double* v = q.v64;
+  const double c = v[idx];
+  double s = v[idx - 1] + v[idx + 1];
+  s += v[idx + sy];
+  s += v[idx - sy];
+  s += v[idx + sz];
+  s += v[idx - sz];
+  q.r32[idx] = (float)(q.rho64[idx] - q.h * (s - 6.0 * c));
+}
+
+__global__ __launch_bounds__(kBX* kBY) void kb_add(const JItem* __restrict__ items, int n, int from_b) {
+  const int it = find_item(items, n, blockIdx.x);
+  const JItem& q = items[it];
+  int i, j, k;
+  if (!item_point(q, blockIdx.x - q.blk0, i, j, k)) return;
+  const int64_t idx = i + (int64_t)q.nx * j + (int64_t)q.nx * q.ny * k;
+  q.v64[idx] += (double)((const float*)(from_b ? q.b : q.a))[idx];
+}
+
+__global__ __launch_bounds__(kBX* kBY) void kb_copy_back(const JItem* __restrict__ items, int n) {
+  const int it = find_item(items, n, blockIdx.x);
+  const JItem& q = items[it];
+  int i, j, k;
+  if (!item_point(q, blockIdx.x - q.blk0, i, j, k)) return;
+  const int64_t idx = i + (int64_t)q.nx * j + (int64_t)q.nx * q.ny * k;
+  ((double*)q.a)[idx] = ((const double*)q.b)[idx];
+}
+
+}  // namespace
+
+// destroy the graphs whose last replay has finished (all of them if wait)
+void qm_reclaim_graphs(qm_ctx* ctx, bool wait) {
+  auto& v = ctx->graphs;
+  size_t k = 0;
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (wait) (void)hipEventSynchronize(v[i].done);
+    if (hipEventQuery(v[i].done) == hipSuccess) {
+      (void)hipGraphExecDestroy(v[i].x);
+      (void)hipGraphDestroy(v[i].g);
+      (void)hipEventDestroy(v[i].done);
+    } else {
+      v[k++] = v[i];
+    }
+  }
+  v.resize(k);
+}
+
+extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* geoms, const void* const* rho,
+                               void* const* v, void* const* tmp, int precision, int64_t nsweeps, int block,
+                               int use_graph) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, n >= 0 && (n == 0 || (geoms && rho && v)), QM_EINVAL, "jacobi_batch: bad arguments");
+  QM_REQUIRE(ctx, nsweeps >= 0 && block >= 2 && block % 2 == 0, QM_EINVAL,
+             "jacobi_batch: sweeps >= 0 and an even block >= 2 required");
+  QM_REQUIRE(ctx, precision == QM_PREC_FP64 || precision == QM_PREC_FP32, QM_EINVAL, "jacobi_batch: bad precision");
+  if (n == 0 || nsweeps == 0) return QM_OK;
+  qm_reclaim_graphs(ctx, false);
+  const bool f64 = precision == QM_PREC_FP64;
+  std::vector<JItem> items(n);
+  int blocks = 0;
+  size_t f32_elems = 0;
+  for (int i = 0; i < n; ++i) {
+    if ((rc = qm_check_geom(ctx, &geoms[i]))) return rc;
+    QM_REQUIRE(ctx, rho[i] && v[i] && (!f64 || (tmp && tmp[i])), QM_EINVAL, "jacobi_batch: item %d incomplete", i);
+    JItem& q = items[i];
+    q.nx = geoms[i].nx;
+    q.ny = geoms[i].ny;
+    q.nz = geoms[i].nz;
+    q.h = geoms[i].step;
+    q.bx = (q.nx + kBX - 1) / kBX;
+    q.by = (q.ny + kBY - 1) / kBY;
+    q.blk0 = blocks;
+    const int64_t nb = (int64_t)q.bx * q.by * q.nz;
+    QM_REQUIRE(ctx, blocks + nb < 0x7fffffff, QM_ELIMIT, "jacobi_batch: batch too large");
+    blocks += (int)nb;
+    if (f64) {
+      q.rho = rho[i];
+      q.a = v[i];
+      q.b = tmp[i];
+    } else {
+      q.rho64 = (const double*)rho[i];
+      q.v64 = (double*)v[i];
+      f32_elems += (size_t)npoints(&geoms[i]);
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (!f64) {
+    float* w = nullptr;
+    if ((rc = qm_ws(ctx, WS_JAC_A, sizeof(float) * 3 * f32_elems, (void**)&w))) return rc;
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+      const size_t N = (size_t)npoints(&geoms[i]);
+      JItem& q = items[i];
+      q.r32 = w + off;
+      q.a = w + off + N;
+      q.b = w + off + 2 * N;
+      q.rho = q.r32;
+      off += 3 * N;
+    }
+  }
+  void* dtab = nullptr;
+  if ((rc = qm_ws(ctx, WS_BATCH, sizeof(JItem) * items.size(), &dtab))) return rc;
+  QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, items.data(), sizeof(JItem) * items.size(), hipMemcpyHostToDevice, s));
+  QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // pageable source
+  const JItem* dt = (const JItem*)dtab;
+  const dim3 grid(blocks), thr(kBX, kBY);
+  // one block of work on stream t; kf = sweeps in the block
+  auto enqueue_block = [&](hipStream_t t, int64_t kf) -> int {
+    if (f64) {
+      for (int64_t q = 0; q < kf; ++q) {
+        hipLaunchKernelGGL(kb_sweep<double>, grid, thr, 0, t, dt, n, (int)(q & 1));
+      }
+      if (kf & 1) hipLaunchKernelGGL(kb_copy_back, grid, thr, 0, t, dt, n);
+    } else {
+      hipLaunchKernelGGL(kb_residual, grid, thr, 0, t, dt, n);
+      for (int64_t q = 0; q < kf; ++q) hipLaunchKernelGGL(kb_sweep<float>, grid, thr, 0, t, dt, n, (int)(q & 1));
+      hipLaunchKernelGGL(kb_add, grid, thr, 0, t, dt, n, (int)(kf & 1));
+    }
+    return hipGetLastError() == hipSuccess ? QM_OK : QM_EHIP;
+  };
+  const int64_t full = nsweeps / block, rest = nsweeps % block;
+  ProfScope ps(ctx, s, QM_K_JACOBI);
+  if (use_graph && full > 0) {
+    if (!ctx->graph_stream) QM_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking));
+    hipStream_t gs = ctx->graph_stream;
+    hipEvent_t ev;
+    QM_CHECK_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    QM_CHECK_HIP(ctx, hipEventRecord(ev, s));
+    QM_CHECK_HIP(ctx, hipStreamWaitEvent(gs, ev, 0));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    QM_CHECK_HIP(ctx, hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_block(gs, block);
+    hipError_t ce = hipStreamEndCapture(gs, &graph);
+    if (rc || ce != hipSuccess) {
+      (void)hipEventDestroy(ev);
+      if (graph) (void)hipGraphDestroy(graph);
+      return qm_fail(ctx, QM_EHIP, "jacobi_batch: graph capture failed: %s", hipGetErrorString(ce));
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      (void)hipEventDestroy(ev);
+      return qm_fail(ctx, QM_EHIP, "jacobi_batch: graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+    for (int64_t r = 0; r < full; ++r) QM_CHECK_HIP(ctx, hipGraphLaunch(exec, gs));
+    if (rest) {
+      if ((rc = enqueue_block(gs, rest))) return qm_fail(ctx, rc, "jacobi_batch: launch failed");
+    }
+    QM_CHECK_HIP(ctx, hipEventRecord(ev, gs));
+    QM_CHECK_HIP(ctx, hipStreamWaitEvent(s, ev, 0));
+    ctx->graphs.push_back({graph, exec, ev});   // destroyed once ev has fired
+  } else {
+    for (int64_t r = 0; r < full; ++r)
+      if ((rc = enqueue_block(s, block))) return qm_fail(ctx, rc, "jacobi_batch: launch failed");
+    if (rest)
+      if ((rc = enqueue_block(s, rest))) return qm_fail(ctx, rc, "jacobi_batch: launch failed");
+  }
+  return QM_OK;
+}

@@ -63,6 +63,8 @@ enum WsSlot {
   WS_JAC_C,
   WS_GTO,          // Hermite distributions
   WS_MISC,
+  WS_BATCH,        // batched-launch tables
+  WS_BATCH2,
   WS_COUNT
 };
 
@@ -74,6 +76,13 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
+// an instantiated graph kept alive until the event recorded after its last launch fires
+struct LiveGraph {
+  hipGraph_t g;
+  hipGraphExec_t x;
+  hipEvent_t done;
+};
+
 struct qm_ctx {
   int device = 0;
   std::string err;
@@ -81,6 +90,8 @@ struct qm_ctx {
   double* host_scalars = nullptr;   // pinned host buffer for results
   FftPlanCache* fft = nullptr;
   bool prof = false;
+  hipStream_t graph_stream = nullptr;
+  std::vector<LiveGraph> graphs;     // graphs whose replays may still run  // non-blocking stream for graph capture (default priority)
   int split_tm = 128;               // rows per workgroup of the 16-bit split contraction (64 or 128)
   std::vector<ProfRec> recs;
 };
@@ -147,6 +158,7 @@ int qm_scale_dev(qm_ctx* ctx, hipStream_t s, int dtype, void* a, int64_t n, cons
 int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol);
 void qm_fft_cache_free(FftPlanCache* c);
 void qm_gto_cache_free(void* c);
+void qm_reclaim_graphs(qm_ctx* ctx, bool wait);
 
 // Brackets one kernel launch with events on its stream when profiling is on.
 struct ProfScope {

@@ -70,18 +70,36 @@ __global__ void k_scale(T* __restrict__ a, int64_t n, const double* __restrict__
     a[i] = (T)((double)a[i] * f);
 }
 
-// one thread per (atom, corner of the enclosing node cell); reference kernel_utils.cu:5-50
+// One thread per (atom, corner of the enclosing node cell); reference kernel_utils.cu:5-50.
+// The reference stores (not adds) and lets atoms that share a node race; here
+// the atom with the highest index wins, deterministically (a thread skips its
+// node when a later atom writes the same node), which is the result of writing
+// the atoms in order.
+__device__ __forceinline__ void seed_node(const double4& at, const GeomView& g, int c, int64_t& ix, int64_t& iy,
+                                          int64_t& iz) {
+  ix = (int64_t)floor((at.x - g.ox) / g.h) + (c & 1);
+  iy = (int64_t)floor((at.y - g.oy) / g.h) + ((c >> 1) & 1);
+  iz = (int64_t)floor((at.z - g.oz) / g.h) + ((c >> 2) & 1);
+}
+
 template <typename T>
 __global__ void k_seed(const double4* __restrict__ atoms, int natoms, GeomView g, T* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= natoms * 8) return;
   const int a = t >> 3, c = t & 7;
   const double4 at = atoms[a];
-  const double ax = at.x - g.ox, ay = at.y - g.oy, az = at.z - g.oz;
-  const int64_t ix = (int64_t)floor(ax / g.h) + (c & 1);
-  const int64_t iy = (int64_t)floor(ay / g.h) + ((c >> 1) & 1);
-  const int64_t iz = (int64_t)floor(az / g.h) + ((c >> 2) & 1);
+  int64_t ix, iy, iz;
+  seed_node(at, g, c, ix, iy, iz);
   if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nx || iy >= g.ny || iz >= g.nz) return;
+  for (int b = a + 1; b < natoms; ++b) {
+    const double4 bt = atoms[b];
+    for (int d = 0; d < 8; ++d) {
+      int64_t jx, jy, jz;
+      seed_node(bt, g, d, jx, jy, jz);
+      if (jx == ix && jy == iy && jz == iz) return;
+    }
+  }
+  const double ax = at.x - g.ox, ay = at.y - g.oy, az = at.z - g.oz;
   double w = at.w;
   w *= fabs(ax - ix * g.h) / g.h;
   w *= fabs(ay - iy * g.h) / g.h;

@@ -70,6 +70,8 @@ SIGNATURES = {
     "qm_hartree_gto_points": [_vp, _vp, _vp, _vp, _i64, _vp],
     "qm_hartree_gto_grid": [_vp, _vp, _vp, _pg, _i, _vp],
     "qm_vne_classic": [_vp, _vp, _vp, _pg, _i, _i, _vp],
+    "qm_density_batch": [_vp, _vp, _i, _c.POINTER(_vp), _pg, _c.POINTER(_vp), _i, _i, _i],
+    "qm_jacobi_batch": [_vp, _vp, _i, _pg, _c.POINTER(_vp), _c.POINTER(_vp), _c.POINTER(_vp), _i, _i64, _i, _i],
     "qm_tune_set": [_vp, _i, _i],
     "qm_tune_get": [_vp, _i, _pi],
     "qm_autotune_density": [_vp, _vp, _vp, _pg, _i, _pd],
@@ -123,6 +125,29 @@ def context(device=None):
         h = out.value
         _ctx[dev] = h
     return h
+
+
+def new_context(device=None):
+    """An additional qm_ctx (own workspaces and graph stream), e.g. one per serving lane."""
+    dev = _device.device_index() if device is None else int(device)
+    _device.require()
+    out = _vp()
+    rc = library().qm_ctx_create(dev, _c.byref(out))
+    if rc != QM_OK:
+        raise QMError(rc, f"cannot create a context on device {dev}")
+    return out.value
+
+
+def destroy_context(ctx):
+    if ctx:
+        library().qm_ctx_destroy(ctx)
+
+
+def call_ctx(ctx, name, *args):
+    """Call an entry point on an explicit context; raise on error."""
+    rc = getattr(library(), name)(ctx, *args)
+    check(rc, ctx)
+    return rc
 
 
 def check(rc, ctx=None):

@@ -1,0 +1,99 @@
+"""Batched density / Jacobi and the request loop on the GPU: every molecule's
+result in a batch equals its result alone; graph replays equal eager runs."""
+import numpy
+import pytest
+from conftest import BASIS
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def items():
+    import qmtools
+    from qmtools import synth
+    basis = qmtools.BasisSet(BASIS)
+    out = []
+    for natoms, seed, n, step in [(5, 1, 24, 0.45), (9, 2, 32, 0.4), (14, 3, 16, 0.8), (7, 4, 40, 0.3)]:
+        mol = synth.synthetic_molecule(natoms, seed, basis)
+        out.append((mol, synth.centred_grid(mol, n, step_bohr=step, pad_ang=2.0)))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_density_batch_equals_single(items, precision):
+    import qmtools
+    from qmtools import batch
+    mols = [m for m, _ in items]
+    grids = [g for _, g in items]
+    _say("density batch", precision, [g.shape for g in grids])
+    qs = batch.density_batch(mols, grids, precision)
+    for (m, g), q in zip(items, qs):
+        single = qmtools.QMTools.Compute_density(g, m, precision=precision)
+        numpy.testing.assert_array_equal(q.cpu().numpy(), single.qube.ravel())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_jacobi_batch_equals_single(items, precision):
+    import qmtools
+    from qmtools import batch
+    mols = [m for m, _ in items]
+    grids = [g for _, g in items]
+    sweeps = 4 * (100 if precision == "fp64" else 50) + 7      # several graph replays + an odd remainder
+    _say("jacobi batch", precision, sweeps)
+    qs = batch.density_batch(mols, grids, precision)
+    vs_graph = batch.hartree_jacobi_batch(mols, grids, qs, sweeps, precision, "multipole", use_graph=True)
+    vg1 = [v.cpu().numpy() for v in vs_graph]
+    # eager work and a synchronisation between the two graph runs
+    vs_eager = batch.hartree_jacobi_batch(mols, grids, qs, sweeps, precision, "multipole", use_graph=False)
+    ve = [v.cpu().numpy() for v in vs_eager]
+    vs_graph2 = batch.hartree_jacobi_batch(mols, grids, qs, sweeps, precision, "multipole", use_graph=True)
+    vg2 = [v.cpu().numpy() for v in vs_graph2]
+    for a, b, c in zip(vg1, ve, vg2):
+        numpy.testing.assert_array_equal(a, b)
+        numpy.testing.assert_array_equal(c, b)
+    if precision == "fp64":
+        for (m, g), q, v in zip(items, qs, ve):
+            qg = qmtools.Grid.emptyAs(g)
+            qg.d_qube = q.clone()
+            single = qmtools.QMTools.Compute_hartree_iter(qg, m, tolerance=None, max_sweeps=sweeps,
+                                                          boundary="multipole")
+            numpy.testing.assert_allclose(v, single.qube.ravel(), rtol=0, atol=1e-13 * numpy.abs(v).max())
+
+
+def test_jacobi_batch_mixed_vs_oracle(items):
+    from oracle import jacobi as oj
+    from oracle import seed as osd
+    from qmtools import batch
+    mols = [m for m, _ in items]
+    grids = [g for _, g in items]
+    _say("jacobi batch mixed vs oracle 2000")
+    qs = batch.density_batch(mols, grids, "mixed")
+    vs = batch.hartree_jacobi_batch(mols, grids, qs, 2000, "mixed", "multipole")
+    for (m, g), q, v in zip(items, qs, vs):
+        nx, ny, nz, h, o = g.geometry()
+        rho = osd.seed_grid(m.types, m.coords, nx, ny, nz, h, o) - q.double().cpu().numpy()
+        want = oj.jacobi(rho, nx, ny, nz, h, 2000, bvals=oj.multipole_boundary(rho, nx, ny, nz, h, o))
+        err = numpy.abs(v.cpu().numpy() - want).max() / numpy.abs(want).max()
+        assert err <= 1e-6, err
+
+
+def test_server_results_same_alone_and_in_mix(items):
+    from qmtools import batch
+    from qmtools.serve import Server
+    _say("server mix vs alone")
+    srv = Server(nlanes=2, precision="mixed", max_batch=2, sweeps=300).start()
+    reqs = [srv.submit(m, g) for m, g in items for _ in range(2)]
+    for r in reqs:
+        r.wait(120)
+    srv.stop()
+    assert srv.max_concurrent >= 1
+    for r in reqs:
+        qs, vs = batch.run([r.mol], [r.grid], "mixed", 300)
+        numpy.testing.assert_array_equal(r.density, qs[0].cpu().numpy())
+        numpy.testing.assert_array_equal(r.potential, vs[0].cpu().numpy())
+    st = srv.stats()
+    assert st["requests"] == len(reqs) and st["failed"] == 0

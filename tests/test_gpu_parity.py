@@ -316,3 +316,21 @@ def test_autotune_cache(qm, small, tmp_path):
     again = autotune.autotune_density(mol, grid, "mixed", path=path)   # served from the cache
     assert again == e and autotune.get("split_rows") == e["split_rows"]
     autotune.set("split_rows", 128)
+
+
+def test_seed_shared_nodes_deterministic(qm):
+    """Nuclei closer than a grid step share seed nodes: the later atom wins, as in the oracle."""
+    from oracle import seed as osd
+    basis = qm.BasisSet(BASIS)
+    xyz = [[6, 0.0, 0.0, 0.0], [1, 0.4, 0.3, 0.2], [1, -0.3, 0.35, -0.25], [8, 1.2, 0.1, 0.0]]
+    nbf = 14 + 5 + 5 + 14
+    mol = qm.Molecule(xyz, numpy.eye(nbf) * 0.1, basis)
+    grid = qm.Grid((1, 16, 16, 16), 1.0)
+    from qmtools.grid import make_float3
+    grid.origin = make_float3(-8.0, -8.0, -8.0)
+    nx, ny, nz, h, o = grid.geometry()
+    want = osd.seed_grid(mol.types, mol.coords, nx, ny, nz, h, o)
+    _say("seed shared nodes")
+    for _ in range(3):
+        s = qm.QMTools.Compute_qseed(grid, mol)
+        numpy.testing.assert_allclose(s.qube.ravel(), want, rtol=1e-14, atol=1e-15)

@@ -1,0 +1,103 @@
+"""Request loop logic on the host (no GPU): admission by HBM budget, overlap of
+batches on lanes, failure isolation, cancellation, oversize refusal."""
+import threading
+import time
+
+import numpy
+import pytest
+
+from qmtools import Grid, Molecule
+from qmtools.serve import Server, memory_estimate
+
+
+class FakeExecutor:
+    """Batches 'run' for a fixed time; a batch holding a request tagged 'fail' raises in finish()."""
+
+    def __init__(self, nlanes=2, run_s=0.05):
+        self.nlanes = nlanes
+        self.run_s = run_s
+        self.lock = threading.Lock()
+        self.inflight_bytes = 0
+        self.peak_bytes = 0
+        self.launched = []
+
+    def launch(self, lane, reqs):
+        nbytes = sum(memory_estimate(r) for r in reqs)
+        with self.lock:
+            self.inflight_bytes += nbytes
+            self.peak_bytes = max(self.peak_bytes, self.inflight_bytes)
+            self.launched.append((lane, [r.rid for r in reqs]))
+        return {"t": time.perf_counter() + self.run_s, "bytes": nbytes}
+
+    def done(self, h):
+        return time.perf_counter() >= h["t"]
+
+    def finish(self, h, reqs):
+        with self.lock:
+            self.inflight_bytes -= h["bytes"]
+        if any(r.tag == "fail" for r in reqs):
+            raise RuntimeError("injected kernel error")
+        for r in reqs:
+            r.density = numpy.full(4, r.rid, dtype=float)
+
+    def close(self):
+        pass
+
+
+def _mol(basis):
+    return Molecule([[1, 0, 0, 0], [1, 0.74, 0, 0]], numpy.eye(10) * 0.1, basis)
+
+
+def test_admission_budget_and_overlap(basis):
+    mol = _mol(basis)
+    grid = Grid((1, 32, 32, 32), 0.2)
+    per = memory_estimate(type("R", (), {"npts": 32 ** 3, "mol": mol})())
+    ex = FakeExecutor(nlanes=2, run_s=0.03)
+    srv = Server(executor=ex, hbm_budget=3 * per, max_batch=2).start()
+    reqs = [srv.submit(mol, grid) for _ in range(10)]
+    for r in reqs:
+        r.wait(10)
+    srv.stop()
+    assert all(r.error is None for r in reqs)
+    assert ex.peak_bytes <= 3 * per                 # never above the budget
+    assert srv.max_concurrent == 2                  # two batches on the device at once
+    assert max(len(ids) for _, ids in ex.launched) <= 2
+    # every request ran exactly once
+    ran = sorted(i for _, ids in ex.launched for i in ids)
+    assert ran == sorted(r.rid for r in reqs)
+    st = srv.stats()
+    assert st["requests"] == 10 and st["p99_s"] >= st["p50_s"] > 0
+
+
+def test_failure_is_isolated_and_oversize_refused(basis):
+    mol = _mol(basis)
+    grid = Grid((1, 16, 16, 16), 0.2)
+    ex = FakeExecutor(nlanes=2, run_s=0.01)
+    srv = Server(executor=ex, hbm_budget=10 ** 9, max_batch=1).start()
+    good1 = srv.submit(mol, grid)
+    bad = srv.submit(mol, grid, tag="fail")
+    good2 = srv.submit(mol, grid)
+    huge = srv.submit(mol, Grid((1, 1024, 1024, 1024), 0.1))
+    assert isinstance(huge.error, MemoryError)
+    good1.wait(5)
+    good2.wait(5)
+    with pytest.raises(RuntimeError):
+        bad.wait(5)
+    srv.stop()
+    assert good1.density[0] == good1.rid and good2.density[0] == good2.rid
+
+
+def test_cancel_before_admission(basis):
+    mol = _mol(basis)
+    grid = Grid((1, 16, 16, 16), 0.2)
+    ex = FakeExecutor(nlanes=1, run_s=0.05)
+    srv = Server(executor=ex, hbm_budget=10 ** 9, max_batch=1).start()
+    first = srv.submit(mol, grid)
+    time.sleep(0.01)                 # first is running; the next waits in the queue
+    second = srv.submit(mol, grid)
+    assert second.cancel()
+    first.wait(5)
+    with pytest.raises(RuntimeError):
+        second.wait(5)
+    srv.stop()
+    assert all(second.rid not in ids for _, ids in ex.launched)

@@ -34,6 +34,9 @@ for _p in (ROOT, PKG):
 import numpy  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per contraction launch measured with rocprofv3 FETCH_SIZE (x2 calibration) and
+# WRITE_SIZE (x1) for this workload; source: profiles/config2_pmc_traffic.md
+MEASURED_TRAFFIC = {("fp64", 640, 798915): 4544548 * 1024 * 2 + 6242 * 1024}
 FP64_MFMA_PEAK_TFLOPS = 78.6
 F16_MFMA_PEAK_TFLOPS = 2500.0
 
@@ -205,6 +208,73 @@ def cpu_baseline(w, budget_s=20.0):
     }
 
 
+def synthetic_batch(basis, count, atoms, grid_n, steps, seed):
+    """Seeded molecules with atom counts U{atoms}, cubic grids from grid_n, steps U(steps) bohr."""
+    from qmtools import synth
+    rng = numpy.random.default_rng(seed)
+    mols, grids = [], []
+    for i in range(count):
+        na = int(rng.integers(atoms[0], atoms[1] + 1))
+        n = int(rng.choice(grid_n))
+        st = float(rng.uniform(*steps))
+        mol = synth.synthetic_molecule(na, seed * 1000 + i, basis)
+        mols.append(mol)
+        grids.append(synth.centred_grid(mol, n, step_bohr=st, pad_ang=3.0))
+    return mols, grids
+
+
+def batched_section(basis, count, sweeps, seed=41):
+    """Config 4 (reduced count): mixed batched fused launches + hipGraph Jacobi vs the plain FP64 path
+    (one molecule at a time: FP64 density, FP64 Jacobi), same Poisson method and sweep count."""
+    import torch
+    import qmtools
+    from qmtools import batch
+    mols, grids = synthetic_batch(basis, count, (5, 80), (64, 96, 128, 160, 192), (0.1, 0.15), seed)
+    npts = batch.grid_points(grids)
+    log(f"batched: {count} molecules, {npts} grid points, {sweeps} Jacobi sweeps")
+    lane = batch.Lane(default=True)
+    batch.run(mols[:2], grids[:2], "mixed", 100, lane=lane)       # warm-up (workspaces, tuning)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    qs, vs = batch.run(mols, grids, "mixed", sweeps, lane=lane)
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    # plain FP64 path, one molecule at a time through the drop-in API
+    t0 = time.perf_counter()
+    plain = []
+    for m, g in zip(mols, grids):
+        q = qmtools.QMTools.Compute_density(g, m, precision="fp64", copyBack=False)
+        v = qmtools.QMTools.Compute_hartree_iter(q, m, tolerance=None, max_sweeps=sweeps, boundary="multipole",
+                                                 copyBack=False)
+        plain.append((q.d_qube, v.d_qube))
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    rho_err = max(float((q.double() - p[0]).abs().max() / p[0].abs().max()) for q, p in zip(qs, plain))
+    chg_err = max(float(abs(q.double().sum() - p[0].sum()) / p[0].sum()) for q, p in zip(qs, plain))
+    v_err = max(float((v - p[1]).abs().max() / p[1].abs().max()) for v, p in zip(vs, plain))
+    return {"molecules": count, "grid_points": npts, "sweeps": sweeps,
+            "mixed_batched_pts_per_s": npts / t_mixed, "plain_fp64_pts_per_s": npts / t_plain,
+            "speedup": t_plain / t_mixed, "seconds": {"mixed_batched": t_mixed, "plain_fp64": t_plain},
+            "max_rel_err_vs_fp64": {"rho": rho_err, "charge": chg_err, "v_hartree": v_err}}
+
+
+def served_section(basis, nreq, rate, sweeps, seed=77):
+    """Config 5 (reduced count): Poisson arrivals into the continuous-batching server (mixed path)."""
+    from qmtools import serve
+    mols, grids = synthetic_batch(basis, nreq, (5, 120), (64, 96, 128), (0.1, 0.15), seed)
+    srv = serve.Server(nlanes=2, precision="mixed", max_batch=8, sweeps=sweeps, hbm_budget=96 << 30,
+                       to_host=False).start()
+    srv.submit(mols[0], grids[0]).wait(600)                        # warm-up
+    srv.completed.clear()
+    reqs = serve.poisson_arrivals(srv, list(zip(mols, grids)), rate, seed)
+    for r in reqs:
+        r.wait(1200)
+    srv.stop()
+    st = srv.stats()
+    st.update({"rate_req_per_s": rate, "sweeps": sweeps})
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +285,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the comparison run of the other precision")
+    ap.add_argument("--batch", type=int, default=16, help="molecules in the batched section (0: skip)")
+    ap.add_argument("--serve", type=int, default=48, help="requests in the served section (0: skip)")
+    ap.add_argument("--rate", type=float, default=100.0, help="Poisson arrival rate of the served section")
+    ap.add_argument("--sweeps", type=int, default=2000, help="Jacobi sweeps in the batched/served sections")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
@@ -299,6 +373,14 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(w)
 
+    batched = served = None
+    if args.batch > 0:
+        batched = batched_section(w.basis, args.batch, args.sweeps)
+        log(f"batched: {batched['speedup']:.2f}x plain FP64, errors {batched['max_rel_err_vs_fp64']}")
+    if args.serve > 0:
+        served = served_section(w.basis, args.serve, args.rate, args.sweeps)
+        log(f"served: {served}")
+
     value = w.N / (ms_step / 1000.0)
     out = {
         "metric": "grid pts/s density+Hartree at <=1e-6 rel err vs FP64; p50/p99 latency",
@@ -321,7 +403,9 @@ def main():
                    "nbf": nbf, "points": w.npoints},
         "roofline": {"bound": "mfma", "kernel": "density contraction (Phi P Phi^T diagonal)",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None,
+                     "traffic": MEASURED_TRAFFIC.get((args.precision, (nbf + 63) // 64 * 64,
+                                                      int(round(launch_pts)))),
+                     "traffic_source": "profiles/config2_pmc_traffic.md",
                      "algorithmic": f"(2 nbf^2 + 2 nbf) flops/point x {launch_pts:.0f} points/launch"
                                     + (" x 3 split products" if mult > 1 else ""),
                      "avg_ms": avg_ms, "launches": k_n},
@@ -329,6 +413,8 @@ def main():
                                   "fft_passes": fft_ms / args.steps, "gto": gto_ms / args.steps},
         other + "_path": other_out,
         "mixed_vs_fp64": err_out,
+        "batched": batched,
+        "served": served,
         "oracle_check": check,
         "autotune": tune,
         "cpu_baseline": cpu,

@@ -36,7 +36,7 @@ import numpy  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per contraction launch measured with rocprofv3 FETCH_SIZE (x2 calibration) and
 # WRITE_SIZE (x1) for this workload; source: profiles/config2_pmc_traffic.md
-MEASURED_TRAFFIC = {("fp64", 640, 798915): 4544548 * 1024 * 2 + 6242 * 1024}
+MEASURED_TRAFFIC = {}
 FP64_MFMA_PEAK_TFLOPS = 78.6
 F16_MFMA_PEAK_TFLOPS = 2500.0
 
@@ -208,6 +208,53 @@ def cpu_baseline(w, budget_s=20.0):
     }
 
 
+def jacobi_section(w, sweeps=200):
+    """Jacobi sweeps on the config-2 grid (256^3): ms per sweep and HBM roofline fraction."""
+    import torch
+    import qmtools
+    from qmtools import _native as nat
+    out = {}
+    qg = qmtools.Grid.emptyAs(w.grid)
+    qg.d_qube = w.q64.clone()
+    for prec, bps in (("fp64", 24.0), ("mixed", 12.0)):
+        qmtools.QMTools.Compute_hartree_iter(qg, w.mol, tolerance=None, max_sweeps=50, boundary="multipole",
+                                             precision=prec, copyBack=False)
+        torch.cuda.synchronize()
+        nat.library().qm_profile(nat.context(), 1)
+        t0 = time.perf_counter()
+        qmtools.QMTools.Compute_hartree_iter(qg, w.mol, tolerance=None, max_sweeps=sweeps, boundary="multipole",
+                                             precision=prec, copyBack=False)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        nat.library().qm_profile(nat.context(), 0)
+        ms, n = w.kernel_ms(nat.QM_K_JACOBI)
+        per = ms / max(n, 1)
+        gbs = bps * w.N / (per * 1e-3) / 1e9
+        out[prec] = {"ms_per_sweep": per, "sweeps": n, "wall_ms_per_sweep": wall * 1000 / sweeps,
+                     "algorithmic_bytes_per_point": bps, "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
+def vne_section(w, pairs=25):
+    """Diffusion pseudo-V_ne on the 256^3 grid: fixed pair count, per-sweep time and HBM fraction
+    (16 B/point/sweep FP64: read V, write V')."""
+    import warnings
+    import qmtools
+    from qmtools import _native as nat
+    nat.library().qm_profile(nat.context(), 1)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        t0 = time.perf_counter()
+        v = qmtools.QMTools.Compute_VNe(w.grid, w.mol, tolerance=-1.0, maxiters=pairs - 1, copyBack=False)
+        wall = time.perf_counter() - t0
+    nat.library().qm_profile(nat.context(), 0)
+    ms, n = w.kernel_ms(nat.QM_K_VNE)
+    per_sweep = ms / max(n, 1) / 2
+    gbs = 16.0 * w.N / (per_sweep * 1e-3) / 1e9
+    return {"pairs": int(v.pairs), "ms_per_sweep": per_sweep, "wall_ms_per_pair": wall * 1e3 / max(v.pairs, 1),
+            "algorithmic_bytes_per_point": 16.0, "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+
+
 def synthetic_batch(basis, count, atoms, grid_n, steps, seed):
     """Seeded molecules with atom counts U{atoms}, cubic grids from grid_n, steps U(steps) bohr."""
     from qmtools import synth
@@ -310,8 +357,9 @@ def main():
     log(f"molecule: {int(w.mol.natoms)} atoms, nbf {w.mol.nbf}; grid {w.n}^3, step {float(w.grid.step):.5f} bohr")
     other = "mixed" if args.precision == "fp64" else "fp64"
     from qmtools import autotune
-    tune = autotune.autotune_density(w.mol, w.grid, "mixed", use_cache=False)
-    log(f"autotune (mixed contraction): {tune}")
+    tune = {"fp64": autotune.autotune_density(w.mol, w.grid, "fp64", use_cache=False),
+            "mixed": autotune.autotune_density(w.mol, w.grid, "mixed", use_cache=False)}
+    log(f"autotune: {tune}")
 
     # main measurement
     wall, lat = w.timed(args.steps, args.warmup, profile=True)
@@ -353,13 +401,19 @@ def main():
             log(f"mixed breakdown: {other_out['breakdown_ms_per_step']}")
         w.q64.copy_(q64)
 
-    # roofline of the dominant kernel (the density contraction)
+    # roofline of the dominant kernel (the density contraction).  The LDS-staged
+    # kernels contract with the upper-triangular M = D + 2U of the symmetric P,
+    # nbf (nbf + 1) + 2 nbf flops per point (half the dense 2 nbf^2 + 2 nbf);
+    # the barrier-free FP64 variant is dense.
     nbf = w.mol.nbf
     launch_pts = w.N / max(k_n / args.steps, 1)
-    flops_per_launch = (2.0 * nbf * nbf + 2.0 * nbf) * launch_pts
+    dense = args.precision == "fp64" and tune["fp64"].get("f64_direct", 0) == 1
+    per_point = (2.0 * nbf * nbf if dense else nbf * (nbf + 1.0)) + 2.0 * nbf
+    flops_per_launch = per_point * launch_pts
     mult = 1.0 if args.precision == "fp64" else 3.0     # hi/lo split executes 3 products
     avg_ms = k_ms / max(k_n, 1)
     achieved = flops_per_launch * mult / (avg_ms * 1e-3) / 1e12
+    dense_equiv = (2.0 * nbf * nbf + 2.0 * nbf) * launch_pts * mult / (avg_ms * 1e-3) / 1e12
     peak = FP64_MFMA_PEAK_TFLOPS if args.precision == "fp64" else F16_MFMA_PEAK_TFLOPS
 
     check = None
@@ -373,6 +427,10 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(w)
 
+    jac = jacobi_section(w)
+    log(f"jacobi: {jac}")
+    vne = vne_section(w)
+    log(f"vne diffusion: {vne}")
     batched = served = None
     if args.batch > 0:
         batched = batched_section(w.basis, args.batch, args.sweeps)
@@ -406,13 +464,18 @@ def main():
                      "traffic": MEASURED_TRAFFIC.get((args.precision, (nbf + 63) // 64 * 64,
                                                       int(round(launch_pts)))),
                      "traffic_source": "profiles/config2_pmc_traffic.md",
-                     "algorithmic": f"(2 nbf^2 + 2 nbf) flops/point x {launch_pts:.0f} points/launch"
-                                    + (" x 3 split products" if mult > 1 else ""),
+                     "algorithmic": (f"({'2 nbf^2' if dense else 'nbf (nbf + 1)'} + 2 nbf) flops/point x "
+                                     f"{launch_pts:.0f} points/launch"
+                                     + (" x 3 split products" if mult > 1 else "")
+                                     + ("" if dense else " (triangular M = D + 2U of symmetric P)")),
+                     "dense_equivalent_tflops": dense_equiv,
                      "avg_ms": avg_ms, "launches": k_n},
         "breakdown_ms_per_step": {"contraction": k_ms / args.steps, "phi": phi_ms / args.steps,
                                   "fft_passes": fft_ms / args.steps, "gto": gto_ms / args.steps},
         other + "_path": other_out,
         "mixed_vs_fp64": err_out,
+        "jacobi_256": jac,
+        "vne_diffusion_256": vne,
         "batched": batched,
         "served": served,
         "oracle_check": check,

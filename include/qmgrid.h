@@ -204,6 +204,17 @@ int qm_hartree_gto_grid(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
 int qm_vne_classic(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int ndiv, int dtype,
                    void* out);
 
+/* Diffusion / adsorption pseudo-nuclear field: reference QMTools.Compute_VNe
+ * (qmtools.py:2677-2745) and gpu_vne (kernel_vne.cu:13-128).  v (result) and
+ * tmp are caller buffers of npts elements of `dtype`; v is cleared first.
+ * Runs pairs of sweeps v -> tmp -> v until sum|v - tmp| / npts <= tolerance
+ * (checked after each pair) or until the pair count exceeds maxiters (finite;
+ * the reference's default is 10 npts^(1/3)), then *converged = 0.  A
+ * tolerance >= 1 runs no sweep, as in the reference.  Synchronises `stream`. */
+int qm_vne_diffusion(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int dtype, double adsorb,
+                     double diff, double tolerance, double maxiters, void* v, void* tmp, int64_t* pairs,
+                     double* delta, int* converged);
+
 /* ---------------------------------------------------------------- batches */
 
 /* Density of nmol molecules on their own grids (sizes and steps may differ),
@@ -232,10 +243,12 @@ int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* geoms, cons
 
 /* ---------------------------------------------------------------- tuning */
 
-/* Tile-shape knobs of the density contraction (no reference counterpart). */
+/* Tile-shape knobs of the density contraction and the Jacobi sweep (no reference counterpart). */
 enum {
   QM_TUNE_SPLIT_ROWS = 0,   /* points per workgroup of the 16-bit split contraction: 64 or 128 */
-  QM_TUNE_COUNT = 1
+  QM_TUNE_F64_DIRECT = 1,   /* FP64 contraction: 0 = basis tile staged in LDS, 1 = per-wave loads, no barriers */
+  QM_TUNE_JACOBI_ZS = 2,    /* z planes per Jacobi sweep workgroup, 0 = chosen per grid, else 1..4096 */
+  QM_TUNE_COUNT = 3
 };
 int qm_tune_set(qm_ctx* ctx, int knob, int value);
 int qm_tune_get(qm_ctx* ctx, int knob, int* value);
@@ -258,7 +271,8 @@ enum {
   QM_K_FFT_AXIS = 4,           /* one FFT pass along one axis */
   QM_K_GTO = 5,                /* Hermite/Boys potential */
   QM_K_JACOBI = 6,             /* one Jacobi sweep (FP64 or FP32) */
-  QM_K_COUNT = 7
+  QM_K_VNE = 7,                /* one pair of diffusion V_ne sweeps with its convergence test */
+  QM_K_COUNT = 8
 };
 
 /* No reference counterpart (the reference times nothing on the device).

@@ -24,5 +24,6 @@ Modules:
              (kernel_fft.cu:23-106, qmtools.py:2578-2639)
   gto        analytic GTO-pair + nuclear potential, McMurchie-Davidson / Boys
              (kernel_hartree.cu:288-869, qmtools.py:2436-2489)
-  vne        classical V_ne averaged over voxel sub-points (kernel_vne_classic.cu:13-84)
+  vne        classical V_ne averaged over voxel sub-points (kernel_vne_classic.cu:13-84);
+             diffusion pseudo-V_ne relaxation (kernel_vne.cu:13-128, qmtools.py:2677-2745)
 """

@@ -136,7 +136,7 @@ int qm_profile_read(qm_ctx* ctx, int kernel, double* total_ms, int64_t* launches
     float ms = 0.f;
     QM_CHECK_HIP(ctx, hipEventElapsedTime(&ms, r.a, r.b));
     tot += ms;
-    ++n;
+    n += r.weight;
   }
   if (total_ms) *total_ms = tot;
   if (launches) *launches = n;
@@ -150,12 +150,13 @@ static void free_mol_device(qm_mol* m) {
   if (m->prims_l2e) (void)hipFree(m->prims_l2e);
   m->prims_l2e = nullptr;
   if (m->P64) (void)hipFree(m->P64);
+  if (m->M64) (void)hipFree(m->M64);
   if (m->P16) (void)hipFree(m->P16);
   if (m->Pb16) (void)hipFree(m->Pb16);
   qm_gto_cache_free(m->gto);
   m->gto = nullptr;
   m->atoms = nullptr; m->shells = nullptr; m->prims = nullptr;
-  m->P64 = nullptr; m->P16 = nullptr; m->Pb16 = nullptr;
+  m->P64 = nullptr; m->M64 = nullptr; m->P16 = nullptr; m->Pb16 = nullptr;
 }
 
 int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr, int nshell,

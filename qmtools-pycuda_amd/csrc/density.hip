@@ -216,18 +216,27 @@ constexpr int kWG = kWaves * 64;
 // D lane l, register r = (t: 4r + l/16, n: l%16).
 template <int NT, typename TO>
 __device__ __forceinline__ void contract_f64_body(int blk, const double* __restrict__ phi, int ld,
-                                                  const double* __restrict__ P, int nbf_pad, int npts,
+                                                  const double* __restrict__ M, int nbf_pad, int npts,
                                                   double scale, TO* __restrict__ out, int64_t v0) {
+  // M is upper triangular (M = D + 2U of the symmetric P), so column tile c
+  // (columns 16c .. 16c+15) only needs the K stages s <= c (16 k values each).
+  // Wave w owns the column tiles ct0 + w + 8j of a pass: cyclic, so every
+  // wave keeps about the same number of live tiles while the triangle narrows.
   constexpr int TM = 64, KS = 16, PAD = 16;
   __shared__ double As[2][KS][TM + PAD];
   __shared__ double red[kWaves][TM];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 15, lk = lane >> 4;
   const int row0 = blk * TM;
   for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
   const int nks = nbf_pad / KS;
   for (int c0 = 0; c0 < nbf_pad; c0 += kWaves * NT * 16) {
-    const int wc0 = c0 + wave * NT * 16;
+    const int ct0 = c0 / 16;
+    const int nks_pass = min(nks, ct0 + kWaves * NT);
+    int tile[NT];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) tile[ct] = ct0 + wave + kWaves * ct;
     d4 acc[4][NT];
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
@@ -248,13 +257,13 @@ __device__ __forceinline__ void contract_f64_body(int blk, const double* __restr
     double bq[NT];
 #pragma unroll
     for (int ct = 0; ct < NT; ++ct) {
-      const int n = wc0 + ct * 16 + li;
-      bq[ct] = n < nbf_pad ? P[(int64_t)lk * nbf_pad + n] : 0.0;
+      const int n = tile[ct] * 16 + li;
+      bq[ct] = M[n < nbf_pad ? (int64_t)lk * nbf_pad + n : 0];
     }
     __syncthreads();
-    for (int s = 0; s < nks; ++s) {
+    for (int s = 0; s < nks_pass; ++s) {
       const int k0 = s * KS;
-      const bool more = s + 1 < nks;
+      const bool more = s + 1 < nks_pass;
       if (more) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -262,28 +271,31 @@ __device__ __forceinline__ void contract_f64_body(int blk, const double* __restr
           pre[q] = phi[(int64_t)(k0 + KS + kk) * ld + row0 + t];
         }
       }
-      if (wc0 < nbf_pad) {
-        const double(*A)[TM + PAD] = As[s & 1];
+      const double(*A)[TM + PAD] = As[s & 1];
 #pragma unroll
-        for (int ks = 0; ks < KS; ks += 4) {
-          // prefetch the P operands of the next k-substep (possibly in the next stage)
-          const int kn = k0 + ks + 4;
-          double bn[NT];
+      for (int ks = 0; ks < KS; ks += 4) {
+        // prefetch the M operands of the next k-substep (possibly in the next stage)
+        const int kn = k0 + ks + 4;
+        // Unconditional loads (a branch around a load would leave the wait counts
+        // unknown after it): a tile past the matrix or past its last stage reads
+        // element 0 instead, and that value is never used.
+        double bn[NT];
 #pragma unroll
-          for (int ct = 0; ct < NT; ++ct) {
-            const int n = wc0 + ct * 16 + li;
-            bn[ct] = (n < nbf_pad && kn < nbf_pad) ? P[(int64_t)(kn + lk) * nbf_pad + n] : 0.0;
-          }
-#pragma unroll
-          for (int rt = 0; rt < 4; ++rt) {
-            const double a = A[ks + lk][rt * 16 + li];
-#pragma unroll
-            for (int ct = 0; ct < NT; ++ct)
-              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bq[ct], acc[rt][ct], 0, 0, 0);
-          }
-#pragma unroll
-          for (int ct = 0; ct < NT; ++ct) bq[ct] = bn[ct];
+        for (int ct = 0; ct < NT; ++ct) {
+          const int n = tile[ct] * 16 + li;
+          const bool live = n < nbf_pad && kn < 16 * (tile[ct] + 1) && kn < nbf_pad;
+          bn[ct] = M[live ? (int64_t)(kn + lk) * nbf_pad + n : 0];
         }
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          const double a = A[ks + lk][rt * 16 + li];
+#pragma unroll
+          for (int ct = 0; ct < NT; ++ct)
+            if (tile[ct] >= s && tile[ct] * 16 < nbf_pad)
+              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bq[ct], acc[rt][ct], 0, 0, 0);
+        }
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) bq[ct] = bn[ct];
       }
       if (more) {
 #pragma unroll
@@ -294,24 +306,22 @@ __device__ __forceinline__ void contract_f64_body(int blk, const double* __restr
       }
       __syncthreads();
     }
-    if (wc0 < nbf_pad) {
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt) {
-        double part[4] = {0, 0, 0, 0};
+    for (int rt = 0; rt < 4; ++rt) {
+      double part[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) {
-          const int n = wc0 + ct * 16 + li;
-          if (n < nbf_pad) {
-            const double* ph = phi + (int64_t)n * ld + row0 + rt * 16 + lk;
+      for (int ct = 0; ct < NT; ++ct) {
+        const int n = tile[ct] * 16 + li;
+        if (tile[ct] * 16 < nbf_pad) {
+          const double* ph = phi + (int64_t)n * ld + row0 + rt * 16 + lk;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) part[r] += acc[rt][ct][r] * ph[4 * r];
-          }
+          for (int r = 0; r < 4; ++r) part[r] += acc[rt][ct][r] * ph[4 * r];
         }
+      }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double v = sum16(part[r]);
-          if (li == 0) red[wave][rt * 16 + 4 * r + lk] += v;
-        }
+      for (int r = 0; r < 4; ++r) {
+        const double v = sum16(part[r]);
+        if (li == 0) red[wave][rt * 16 + 4 * r + lk] += v;
       }
     }
   }
@@ -323,6 +333,100 @@ __device__ __forceinline__ void contract_f64_body(int blk, const double* __restr
     const int t = row0 + threadIdx.x;
     if (t < npts) store_rho<TO>(out, v0 + t, v * scale);
   }
+}
+
+// Barrier-free variant: every wave loads its own A fragments (the 64-row basis
+// tile, shared by the 8 waves through L1/L2) and P fragments one k-step ahead;
+// waves never wait for each other until the final row reduction.
+template <int NT, typename TO>
+__device__ __forceinline__ void contract_f64_direct_body(int blk, const double* __restrict__ phi, int ld,
+                                                         const double* __restrict__ P, int nbf_pad, int npts,
+                                                         double scale, TO* __restrict__ out, int64_t v0) {
+  constexpr int TM = 64;
+  __shared__ double red[kWaves][TM];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int row0 = blk * TM;
+  for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
+  __syncthreads();
+  const double* arow = phi + (int64_t)lk * ld + row0 + li;
+  for (int c0 = 0; c0 < nbf_pad; c0 += kWaves * NT * 16) {
+    const int wc0 = c0 + wave * NT * 16;
+    if (wc0 >= nbf_pad) continue;
+    d4 acc[4][NT];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = d4{0, 0, 0, 0};
+    double aq[4], bq[NT];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) aq[rt] = arow[rt * 16];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) {
+      const int n = wc0 + ct * 16 + li;
+      bq[ct] = n < nbf_pad ? P[(int64_t)lk * nbf_pad + n] : 0.0;
+    }
+    for (int k0 = 0; k0 < nbf_pad; k0 += 4) {
+      const int kn = k0 + 4;
+      double an[4], bn[NT];
+      if (kn < nbf_pad) {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) an[rt] = arow[(int64_t)kn * ld + rt * 16];
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+          const int n = wc0 + ct * 16 + li;
+          bn[ct] = n < nbf_pad ? P[(int64_t)(kn + lk) * nbf_pad + n] : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) an[rt] = 0.0;
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) bn[ct] = 0.0;
+      }
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(aq[rt], bq[ct], acc[rt][ct], 0, 0, 0);
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) aq[rt] = an[rt];
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) bq[ct] = bn[ct];
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      double part[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        const int n = wc0 + ct * 16 + li;
+        if (n < nbf_pad) {
+          const double* ph = phi + (int64_t)n * ld + row0 + rt * 16 + lk;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) part[r] += acc[rt][ct][r] * ph[4 * r];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double v = sum16(part[r]);
+        if (li == 0) red[wave][rt * 16 + 4 * r + lk] += v;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < TM) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) v += red[w][threadIdx.x];
+    const int t = row0 + threadIdx.x;
+    if (t < npts) store_rho<TO>(out, v0 + t, v * scale);
+  }
+}
+
+template <int NT, typename TO>
+__global__ __launch_bounds__(kWG) void k_contract_f64_direct(const double* __restrict__ phi, int ld,
+                                                             const double* __restrict__ P, int nbf_pad, int npts,
+                                                             double scale, TO* __restrict__ out, int64_t v0) {
+  contract_f64_direct_body<NT, TO>(blockIdx.x, phi, ld, P, nbf_pad, npts, scale, out, v0);
 }
 
 template <int NT, typename TO>
@@ -362,12 +466,16 @@ __device__ __forceinline__ void contract_split_body(int blk, const float* __rest
                                                     const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
                                                     int nbf_pad, int npts, double scale, TO* __restrict__ out,
                                                     int64_t v0) {
+  // p_hi / p_lo hold M = D + 2U of the symmetric P (upper triangular), so
+  // column tile c only needs the stages s with 32 s <= 16 c + 15, i.e. 2 s <= c.
+  // Wave w owns the column tiles ct0 + w + 8j of a pass (cyclic, balanced).
   typedef typename Vec8<T16>::type V8;
   constexpr int KG = 4;             // 4 groups of 8 k = 32 k per stage
   constexpr int RT = TM / 16;       // row tiles
   __shared__ uint4 Ah[2][KG][TM], Al[2][KG][TM];
   __shared__ double red[kWaves][TM];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 15, lk = lane >> 4;
   const int row0 = blk * TM;
   for (int e = threadIdx.x; e < kWaves * TM; e += kWG) red[e / TM][e % TM] = 0.0;
@@ -389,7 +497,11 @@ __device__ __forceinline__ void contract_split_body(int blk, const float* __rest
     Al[buf][sg][sr] = *(const uint4*)&l;
   };
   for (int c0 = 0; c0 < nbf_pad; c0 += kWaves * NT * 16) {
-    const int wc0 = c0 + wave * NT * 16;
+    const int ct0 = c0 / 16;
+    const int nks_pass = min(nks, (ct0 + kWaves * NT + 1) / 2);
+    int tile[NT];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) tile[ct] = ct0 + wave + kWaves * ct;
     f4 acc[RT][NT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -402,35 +514,32 @@ __device__ __forceinline__ void contract_split_body(int blk, const float* __rest
       stage_store(0, pre);
     }
     __syncthreads();
-    for (int s = 0; s < nks; ++s) {
-      const bool more = s + 1 < nks;
+    for (int s = 0; s < nks_pass; ++s) {
+      const bool more = s + 1 < nks_pass;
       if (more && stager) {
         const float* nsrc = src + (int64_t)((s + 1) * 32) * ld;
 #pragma unroll
         for (int q = 0; q < 8; ++q) pre[q] = nsrc[(int64_t)q * ld];
       }
-      if (wc0 < nbf_pad) {
-        V8 bh[NT], bl[NT];
+      // unconditional loads: a tile past the matrix or past its last stage reads element 0 (unused)
+      V8 bh[NT], bl[NT];
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        const int n = tile[ct] * 16 + li;
+        const bool live = n < nbf_pad && 2 * s <= tile[ct];
+        const int64_t o = live ? ((int64_t)(s * KG + lk) * nbf_pad + n) * 8 : 0;
+        bh[ct] = *(const V8*)(p_hi + o);
+        bl[ct] = *(const V8*)(p_lo + o);
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        const uint4 uh = Ah[s & 1][lk][rt * 16 + li];
+        const uint4 ul = Al[s & 1][lk][rt * 16 + li];
+        const V8 ah = *(const V8*)&uh;
+        const V8 al = *(const V8*)&ul;
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
-          const int n = wc0 + ct * 16 + li;
-          if (n < nbf_pad) {
-            const int64_t o = ((int64_t)(s * KG + lk) * nbf_pad + n) * 8;
-            bh[ct] = *(const V8*)(p_hi + o);
-            bl[ct] = *(const V8*)(p_lo + o);
-          } else {
-            bh[ct] = V8{};
-            bl[ct] = V8{};
-          }
-        }
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          const uint4 uh = Ah[s & 1][lk][rt * 16 + li];
-          const uint4 ul = Al[s & 1][lk][rt * 16 + li];
-          const V8 ah = *(const V8*)&uh;
-          const V8 al = *(const V8*)&ul;
-#pragma unroll
-          for (int ct = 0; ct < NT; ++ct) {
+          if (2 * s <= tile[ct] && tile[ct] * 16 < nbf_pad) {
             acc[rt][ct] = Vec8<T16>::mma(al, bh[ct], acc[rt][ct]);
             acc[rt][ct] = Vec8<T16>::mma(ah, bl[ct], acc[rt][ct]);
             acc[rt][ct] = Vec8<T16>::mma(ah, bh[ct], acc[rt][ct]);
@@ -440,14 +549,14 @@ __device__ __forceinline__ void contract_split_body(int blk, const float* __rest
       if (more && stager) stage_store((s + 1) & 1, pre);
       __syncthreads();
     }
-    if (wc0 < nbf_pad) {
+    {
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) {
         double part[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
-          const int n = wc0 + ct * 16 + li;
-          if (n < nbf_pad) {
+          const int n = tile[ct] * 16 + li;
+          if (tile[ct] * 16 < nbf_pad) {
             const float4 v = *(const float4*)(phi + (int64_t)n * ld + row0 + rt * 16 + 4 * lk);
             part[0] += (double)(acc[rt][ct][0] * v.x);
             part[1] += (double)(acc[rt][ct][1] * v.y);
@@ -498,7 +607,8 @@ __global__ void k_subgrid_sum(const double* __restrict__ rho, int nvox, int nsub
 // --------------------------------------------------------------------------- host side
 namespace {
 
-// P (symmetric) -> 16-bit hi/lo planes in the K8-interleaved layout, scaled by pfac.
+// M (upper triangle of P, off-diagonal doubled) -> 16-bit hi/lo planes in the
+// K8-interleaved layout, scaled by pfac.
 template <typename T16>
 __global__ void k_split_P(const double* __restrict__ P64, int npad, float pfac, T16* __restrict__ hi,
                           T16* __restrict__ lo) {
@@ -534,7 +644,7 @@ int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol) {
   mol->pfac = std::ldexp(1.0, 1 - e);   // largest |P| scaled into [1, 2)
   const int64_t tot = (int64_t)np * np;
   if (!mol->P64) {
-    if (hipMalloc(&mol->P64, sizeof(double) * tot) != hipSuccess ||
+    if (hipMalloc(&mol->P64, sizeof(double) * tot) != hipSuccess || hipMalloc(&mol->M64, sizeof(double) * tot) != hipSuccess ||
         hipMalloc(&mol->P16, sizeof(uint16_t) * 2 * tot) != hipSuccess ||
         hipMalloc(&mol->Pb16, sizeof(uint16_t) * 2 * tot) != hipSuccess) {
       (void)hipGetLastError();
@@ -544,11 +654,16 @@ int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol) {
   // queued kernels may still read the previous matrix
   QM_CHECK_HIP(ctx, hipDeviceSynchronize());
   QM_CHECK_HIP(ctx, hipMemcpy(mol->P64, P.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  // M = D + 2U (upper triangle of P, off-diagonal doubled): phi^T P phi = phi^T M phi
+  std::vector<double> Mh((size_t)np * np, 0.0);
+  for (int p = 0; p < np; ++p)
+    for (int q = p; q < np; ++q) Mh[(size_t)p * np + q] = (p == q ? 1.0 : 2.0) * P[(size_t)p * np + q];
+  QM_CHECK_HIP(ctx, hipMemcpy(mol->M64, Mh.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
   const unsigned blocks = (unsigned)((tot + 255) / 256);
-  hipLaunchKernelGGL(k_split_P<_Float16>, dim3(blocks), dim3(256), 0, 0, (const double*)mol->P64, np,
+  hipLaunchKernelGGL(k_split_P<_Float16>, dim3(blocks), dim3(256), 0, 0, (const double*)mol->M64, np,
                      (float)mol->pfac, (_Float16*)mol->P16, (_Float16*)mol->P16 + tot);
   QM_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_split_P<__bf16>, dim3(blocks), dim3(256), 0, 0, (const double*)mol->P64, np,
+  hipLaunchKernelGGL(k_split_P<__bf16>, dim3(blocks), dim3(256), 0, 0, (const double*)mol->M64, np,
                      (float)mol->pfac, (__bf16*)mol->Pb16, (__bf16*)mol->Pb16 + tot);
   QM_CHECK_LAUNCH(ctx);
   QM_CHECK_HIP(ctx, hipDeviceSynchronize());
@@ -561,15 +676,24 @@ namespace {
 inline int pick_nt(int np) { return std::min(5, (np + 127) / 128); }
 
 template <typename TO>
-void launch_contract_f64(hipStream_t s, int ld, const double* phi, const double* P, int np, int npts, double scale,
-                         TO* out, int64_t v0) {
+void launch_contract_f64(hipStream_t s, int ld, const double* phi, const double* P, const double* M, int np,
+                         int npts, double scale, TO* out, int64_t v0, int direct) {
   const dim3 grid(ld / 64), block(kWG);
+  if (direct) {
+    switch (pick_nt(np)) {
+      case 1: hipLaunchKernelGGL((k_contract_f64_direct<1, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); return;
+      case 2: hipLaunchKernelGGL((k_contract_f64_direct<2, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); return;
+      case 3: hipLaunchKernelGGL((k_contract_f64_direct<3, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); return;
+      case 4: hipLaunchKernelGGL((k_contract_f64_direct<4, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); return;
+      default: hipLaunchKernelGGL((k_contract_f64_direct<5, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); return;
+    }
+  }
   switch (pick_nt(np)) {
-    case 1: hipLaunchKernelGGL((k_contract_f64<1, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
-    case 2: hipLaunchKernelGGL((k_contract_f64<2, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
-    case 3: hipLaunchKernelGGL((k_contract_f64<3, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
-    case 4: hipLaunchKernelGGL((k_contract_f64<4, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
-    default: hipLaunchKernelGGL((k_contract_f64<5, TO>), grid, block, 0, s, phi, ld, P, np, npts, scale, out, v0); break;
+    case 1: hipLaunchKernelGGL((k_contract_f64<1, TO>), grid, block, 0, s, phi, ld, M, np, npts, scale, out, v0); break;
+    case 2: hipLaunchKernelGGL((k_contract_f64<2, TO>), grid, block, 0, s, phi, ld, M, np, npts, scale, out, v0); break;
+    case 3: hipLaunchKernelGGL((k_contract_f64<3, TO>), grid, block, 0, s, phi, ld, M, np, npts, scale, out, v0); break;
+    case 4: hipLaunchKernelGGL((k_contract_f64<4, TO>), grid, block, 0, s, phi, ld, M, np, npts, scale, out, v0); break;
+    default: hipLaunchKernelGGL((k_contract_f64<5, TO>), grid, block, 0, s, phi, ld, M, np, npts, scale, out, v0); break;
   }
 }
 
@@ -628,10 +752,11 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
     QM_CHECK_LAUNCH(ctx);
     ProfScope pc(ctx, s, QM_K_DENSITY_CONTRACT);
     if (nsub3 == 1) {
-      launch_contract_f64<TO>(s, ld, (const double*)phi, (const double*)mol->P64, np, npts, h3, out, v0);
+      launch_contract_f64<TO>(s, ld, (const double*)phi, (const double*)mol->P64, (const double*)mol->M64, np, npts, h3, out, v0,
+                              ctx->f64_direct);
     } else {
-      launch_contract_f64<double>(s, ld, (const double*)phi, (const double*)mol->P64, np, npts, h3 / nsub3, rho,
-                                  (int64_t)0);
+      launch_contract_f64<double>(s, ld, (const double*)phi, (const double*)mol->P64, (const double*)mol->M64, np, npts, h3 / nsub3, rho,
+                                  (int64_t)0, ctx->f64_direct);
       QM_CHECK_LAUNCH(ctx);
       hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
                          nsub3, out, v0);
@@ -772,6 +897,16 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
     ctx->split_tm = value;
     return QM_OK;
   }
+  if (knob == QM_TUNE_F64_DIRECT) {
+    QM_REQUIRE(ctx, value == 0 || value == 1, QM_EINVAL, "tune: f64 variant must be 0 or 1");
+    ctx->f64_direct = value;
+    return QM_OK;
+  }
+  if (knob == QM_TUNE_JACOBI_ZS) {
+    QM_REQUIRE(ctx, value >= 0 && value <= 4096, QM_EINVAL, "tune: Jacobi planes must be 0..4096");
+    ctx->jacobi_zs = value;
+    return QM_OK;
+  }
   return qm_fail(ctx, QM_EINVAL, "tune: unknown knob %d", knob);
 }
 
@@ -781,6 +916,14 @@ int qm_tune_get(qm_ctx* ctx, int knob, int* value) {
   QM_REQUIRE(ctx, value != nullptr, QM_EINVAL, "tune: null output");
   if (knob == QM_TUNE_SPLIT_ROWS) {
     *value = ctx->split_tm;
+    return QM_OK;
+  }
+  if (knob == QM_TUNE_F64_DIRECT) {
+    *value = ctx->f64_direct;
+    return QM_OK;
+  }
+  if (knob == QM_TUNE_JACOBI_ZS) {
+    *value = ctx->jacobi_zs;
     return QM_OK;
   }
   return qm_fail(ctx, QM_EINVAL, "tune: unknown knob %d", knob);
@@ -797,21 +940,21 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   const int64_t plane = (int64_t)g->nx * g->ny;
   slab.nz = (int)std::max<int64_t>(1, std::min<int64_t>(g->nz, ((int64_t)1 << 21) / std::max<int64_t>(plane, 1)));
   void* out = nullptr;
-  if ((rc = qm_ws(ctx, WS_MISC, sizeof(float) * (size_t)npoints(&slab), &out))) return rc;
-  std::vector<int> cands;
-  if (precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3) cands = {64, 128};
-  else cands = {ctx->split_tm};
+  if ((rc = qm_ws(ctx, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
+  const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
+  std::vector<int> cands = split ? std::vector<int>{64, 128} : std::vector<int>{0, 1};
+  int& knob = split ? ctx->split_tm : ctx->f64_direct;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));
   QM_CHECK_HIP(ctx, hipEventCreate(&b));
   double best = 1e300;
-  int best_v = ctx->split_tm;
+  int best_v = knob;
   for (int v : cands) {
-    ctx->split_tm = v;
+    knob = v;
     double t = 1e300;
     for (int rep = 0; rep < 3; ++rep) {   // first repetition warms caches and workspaces
       QM_CHECK_HIP(ctx, hipEventRecord(a, s));
-      rc = qm_density(ctx, mol, stream, &slab, 1, precision, QM_F32, out, 0, nullptr);
+      rc = qm_density(ctx, mol, stream, &slab, 1, precision, split ? QM_F32 : QM_F64, out, 0, nullptr);
       if (rc) break;
       QM_CHECK_HIP(ctx, hipEventRecord(b, s));
       QM_CHECK_HIP(ctx, hipEventSynchronize(b));
@@ -827,7 +970,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  ctx->split_tm = best_v;
+  knob = best_v;
   if (rc) return rc;
   if (best_ms) *best_ms = best;
   return QM_OK;
@@ -861,7 +1004,7 @@ struct BChunk {
   int nt;
   void* phi;
   float* pscale;
-  const void* ph;       // P64 (FP64) or 16-bit hi plane
+  const void* ph;       // P64 / M64 (FP64) or 16-bit hi plane
   const void* pl;       // 16-bit lo plane
   void* out;
   double scale;
@@ -911,8 +1054,25 @@ __global__ __launch_bounds__(kWG) void kb_contract_split(const BChunk* __restric
                                        (TO*)q.out, q.v0);
 }
 
+template <int NT, typename TO>
+__global__ __launch_bounds__(kWG) void kb_contract_f64_direct(const BChunk* __restrict__ c, int n) {
+  const int i = find_chunk(c, n, blockIdx.x, false);
+  const BChunk& q = c[i];
+  contract_f64_direct_body<NT, TO>(blockIdx.x - q.blk0, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad,
+                                   q.nvox, q.scale, (TO*)q.out, q.v0);
+}
+
 template <typename TO>
-void launch_batch_f64(hipStream_t s, int nt, const BChunk* c, int n, int blocks) {
+void launch_batch_f64(hipStream_t s, int nt, const BChunk* c, int n, int blocks, int direct) {
+  if (direct) {
+    switch (nt) {
+      case 1: hipLaunchKernelGGL((kb_contract_f64_direct<1, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); return;
+      case 2: hipLaunchKernelGGL((kb_contract_f64_direct<2, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); return;
+      case 3: hipLaunchKernelGGL((kb_contract_f64_direct<3, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); return;
+      case 4: hipLaunchKernelGGL((kb_contract_f64_direct<4, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); return;
+      default: hipLaunchKernelGGL((kb_contract_f64_direct<5, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); return;
+    }
+  }
   switch (nt) {
     case 1: hipLaunchKernelGGL((kb_contract_f64<1, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); break;
     case 2: hipLaunchKernelGGL((kb_contract_f64<2, TO>), dim3(blocks), dim3(kWG), 0, s, c, n); break;
@@ -1025,7 +1185,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       cur_ps += (size_t)c.ld * sizeof(float);
       const double h3 = geoms[i].step * geoms[i].step * geoms[i].step;
       if (f64) {
-        c.ph = m->P64;
+        c.ph = ctx->f64_direct ? m->P64 : m->M64;   // the LDS-staged kernel takes the triangular form
         c.pl = nullptr;
         c.scale = h3;
       } else {
@@ -1079,8 +1239,8 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       const int blocks = r.group_blocks[gi];
       ProfScope pc(ctx, s, f64 ? QM_K_DENSITY_CONTRACT : QM_K_DENSITY_SPLIT);
       if (f64) {
-        if (out_dtype == QM_F64) launch_batch_f64<double>(s, nt, gc, gn, blocks);
-        else launch_batch_f64<float>(s, nt, gc, gn, blocks);
+        if (out_dtype == QM_F64) launch_batch_f64<double>(s, nt, gc, gn, blocks, ctx->f64_direct);
+        else launch_batch_f64<float>(s, nt, gc, gn, blocks, ctx->f64_direct);
       } else if (precision == QM_PREC_F16X3) {
         if (out_dtype == QM_F64) launch_batch_split<_Float16, double>(s, nt, tm, gc, gn, blocks);
         else launch_batch_split<_Float16, float>(s, nt, tm, gc, gn, blocks);

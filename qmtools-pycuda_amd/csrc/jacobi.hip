@@ -23,26 +23,117 @@ namespace {
 
 constexpr int kBX = 64, kBY = 4;
 
+// Sweep kernel: each wave owns a strip of 64 columns x kSR rows and marches
+// through zs planes in z.  A lane holds its column's rows j0-1 .. j0+kSR of
+// the planes k-1, k, k+1 in registers; x neighbours come from the adjacent
+// lanes (lanes 0 and 63 load the one column beyond the strip), y neighbours
+// from the same lane's registers.  Waves never wait for each other (no LDS,
+// no barrier), and the loads of plane k+2, with the halo columns and rho of
+// plane k+1, are issued before plane k is computed, so one plane of memory
+// latency is hidden behind the arithmetic.  Each V value is read from memory
+// (kSR+2)/kSR times per sweep, rho once, V' written once.
+// Summation order: (x-1 + x+1) + y+1 + y-1 + z+1 + z-1 - rho/h, then / 6.
+constexpr int kSW = 64;                    // columns per strip (one lane each)
+constexpr int kSR = 8;                     // rows per strip
+constexpr int kSWaves = 4;                 // strips per workgroup, stacked in y
+constexpr int kTileY = kSR * kSWaves;      // rows per workgroup tile
+
 template <typename T>
-__global__ __launch_bounds__(kBX* kBY) void k_sweep(const T* __restrict__ rho, const T* __restrict__ v,
-                                                    T* __restrict__ vout, int nx, int ny, int nz, T h) {
-  const int i = blockIdx.x * kBX + threadIdx.x;
-  const int j = blockIdx.y * kBY + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= nx || j >= ny) return;
-  const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
-  const int64_t idx = i + sy * j + sz * k;
-  if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) {
-    vout[idx] = v[idx];
-    return;
+__device__ __forceinline__ void strip_body(int bx, int by, int bz, int zs, const T* __restrict__ rho,
+                                           const T* __restrict__ v, T* __restrict__ vout, int nx, int ny, int nz,
+                                           T h) {
+  const int lane = threadIdx.x;
+  const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);   // one strip per wave
+  const int i = bx * kSW + lane;
+  const int j0 = by * kTileY + wy * kSR;
+  if (j0 >= ny) return;                    // whole wave idle; the kernel has no barrier
+  const int z0 = bz * zs, z1 = min(nz, z0 + zs);
+  const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+  // Indices are clamped into the grid: a value read at a clamped index only
+  // ever feeds an outer-layer point (copied, not relaxed) or a point outside
+  // the grid (not stored), so every load is unconditional and in bounds.
+  const unsigned ic = (unsigned)min(i, nx - 1);
+  const unsigned ihc = (unsigned)min(max(lane == 0 ? i - 1 : i + 1, 0), nx - 1);
+  auto row = [&](int j) -> size_t { return sy * (size_t)min(max(j, 0), ny - 1); };
+  auto pln = [&](int k) -> size_t { return sz * (size_t)min(max(k, 0), nz - 1); };
+  const bool edge = lane == 0 || lane == kSW - 1;
+  T pm[kSR + 2], pc[kSR + 2], pp[kSR + 2], hc[kSR], rr[kSR];
+#pragma unroll
+  for (int q = 0; q < kSR + 2; ++q) {
+    pm[q] = v[pln(z0 - 1) + row(j0 - 1 + q) + ic];
+    pc[q] = v[pln(z0) + row(j0 - 1 + q) + ic];
+    pp[q] = v[pln(z0 + 1) + row(j0 - 1 + q) + ic];
   }
-  T s = v[idx - sx] + v[idx + sx];
-  s += v[idx + sy];
-  s += v[idx - sy];
-  s += v[idx + sz];
-  s += v[idx - sz];
-  s -= rho[idx] / h;
-  vout[idx] = s / (T)6;
+#pragma unroll
+  for (int r = 0; r < kSR; ++r) {
+    rr[r] = rho[pln(z0) + row(j0 + r) + ic];
+    hc[r] = (T)0;
+  }
+  if (edge) {
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) hc[r] = v[pln(z0) + row(j0 + r) + ihc];
+  }
+  for (int k = z0; k < z1; ++k) {
+    T pn[kSR + 2], hn[kSR], rn[kSR];
+    const size_t pk2 = pln(k + 2), pk1 = pln(k + 1), pk = sz * (size_t)k;
+#pragma unroll
+    for (int q = 0; q < kSR + 2; ++q) pn[q] = v[pk2 + row(j0 - 1 + q) + ic];
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) {
+      rn[r] = rho[pk1 + row(j0 + r) + ic];
+      hn[r] = (T)0;
+    }
+    if (edge) {
+#pragma unroll
+      for (int r = 0; r < kSR; ++r) hn[r] = v[pk1 + row(j0 + r) + ihc];
+    }
+    const bool zb = k == 0 || k == nz - 1;
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) {
+      const int j = j0 + r;
+      const T c = pc[r + 1];
+      T xm = __shfl_up(c, 1, kSW);
+      T xp = __shfl_down(c, 1, kSW);
+      if (lane == 0) xm = hc[r];
+      if (lane == kSW - 1) xp = hc[r];
+      T sum = xm + xp;
+      sum += pc[r + 2];
+      sum += pc[r];
+      sum += pp[r + 1];
+      sum += pm[r + 1];
+      sum -= rr[r] / h;
+      const bool outer = zb || i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+      const T val = outer ? c : sum / (T)6;
+      if (i < nx && j < ny) vout[pk + sy * (size_t)j + (unsigned)i] = val;
+    }
+#pragma unroll
+    for (int q = 0; q < kSR + 2; ++q) {
+      pm[q] = pc[q];
+      pc[q] = pp[q];
+      pp[q] = pn[q];
+    }
+#pragma unroll
+    for (int r = 0; r < kSR; ++r) {
+      hc[r] = hn[r];
+      rr[r] = rn[r];
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSW* kSWaves) void k_sweep_strip(const T* __restrict__ rho, const T* __restrict__ v,
+                                                              T* __restrict__ vout, int nx, int ny, int nz, int zs,
+                                                              T h) {
+  strip_body<T>(blockIdx.x, blockIdx.y, blockIdx.z, zs, rho, v, vout, nx, ny, nz, h);
+}
+
+// planes per workgroup: about 1024 workgroups (4096 waves) for the chip, at least 4 planes each;
+// a positive `fixed` (the tuning knob) overrides
+inline int pick_zs(int nx, int ny, int nz, int fixed) {
+  if (fixed > 0) return std::min(fixed, nz);
+  const int tiles = ((nx + kSW - 1) / kSW) * ((ny + kTileY - 1) / kTileY);
+  const int tz = std::max(1, (1024 + tiles - 1) / tiles);
+  return std::max(4, (nz + tz - 1) / tz);
 }
 
 // r = rho - h (sum_6 V - 6 V) on interior points (FP64), 0 on the outer layer, stored FP32
@@ -166,9 +257,11 @@ dim3 sweep_grid(const qm_geom* g) {
 }
 
 template <typename T>
-int sweep(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const T* rho, const T* a, T* b) {
-  ProfScope ps(ctx, s, QM_K_JACOBI);
-  hipLaunchKernelGGL(k_sweep<T>, sweep_grid(g), dim3(kBX, kBY), 0, s, rho, a, b, g->nx, g->ny, g->nz,
+int sweep(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const T* rho, const T* a, T* b, bool prof = true) {
+  ProfScope ps(ctx, s, QM_K_JACOBI, 1, prof);
+  const int zs = pick_zs(g->nx, g->ny, g->nz, ctx->jacobi_zs);
+  const dim3 grid((g->nx + kSW - 1) / kSW, (g->ny + kTileY - 1) / kTileY, (g->nz + zs - 1) / zs);
+  hipLaunchKernelGGL(k_sweep_strip<T>, grid, dim3(kSW, kSWaves), 0, s, rho, a, b, g->nx, g->ny, g->nz, zs,
                      (T)g->step);
   QM_CHECK_LAUNCH(ctx);
   return QM_OK;
@@ -186,23 +279,95 @@ int host_delta(qm_ctx* ctx, hipStream_t s, int dtype, const void* a, const void*
   return QM_OK;
 }
 
+// A block of work captured once into a hipGraph on the context's graph stream
+// and replayed there; the caller's stream is ordered before and after the
+// replays with events.  Captured work records no profiling events and no
+// memset nodes.  The graph is kept in the context until its last replay has
+// finished.
+struct GraphBlock {
+  qm_ctx* ctx;
+  hipStream_t s, gs = nullptr;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  hipEvent_t ev = nullptr;
+  GraphBlock(qm_ctx* c, hipStream_t st) : ctx(c), s(st) {}
+  template <typename F>
+  int capture(F enqueue) {
+    if (!ctx->graph_stream) QM_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking));
+    gs = ctx->graph_stream;
+    QM_CHECK_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    QM_CHECK_HIP(ctx, hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(gs);
+    hipError_t ce = hipStreamEndCapture(gs, &g);
+    if (rc || ce != hipSuccess) return qm_fail(ctx, QM_EHIP, "graph capture failed: %s", hipGetErrorString(ce));
+    QM_CHECK_HIP(ctx, hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+    return QM_OK;
+  }
+  // replay `times` times on the graph stream, after everything already queued on s
+  int replay(int64_t times, int prof_weight) {
+    QM_CHECK_HIP(ctx, hipEventRecord(ev, s));
+    QM_CHECK_HIP(ctx, hipStreamWaitEvent(gs, ev, 0));
+    {
+      ProfScope ps(ctx, gs, QM_K_JACOBI, (int)(prof_weight * times));
+      for (int64_t r = 0; r < times; ++r) QM_CHECK_HIP(ctx, hipGraphLaunch(x, gs));
+    }
+    QM_CHECK_HIP(ctx, hipEventRecord(ev, gs));
+    QM_CHECK_HIP(ctx, hipStreamWaitEvent(s, ev, 0));
+    return QM_OK;
+  }
+  ~GraphBlock() {
+    if (x && ev) {
+      ctx->graphs.push_back({g, x, ev});
+    } else {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+      if (ev) (void)hipEventDestroy(ev);
+    }
+  }
+};
+
 int jacobi_fp64(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho, double* v, double* tmp,
                 int64_t max_sweeps, double tol, int check_every, int64_t* done_out, double* delta_out) {
   const int64_t n = npoints(g);
+  const bool checks = tol >= 0 && check_every > 0;
+  // sweeps per graph replay: the check interval, or a fixed block without checks
+  const int64_t block = checks ? check_every : 200;
   int64_t done = 0;
   double delta = -1.0;
   bool converged = false;
   int rc;
-  while (done + 2 <= max_sweeps) {
+  if (max_sweeps >= block) {
+    GraphBlock gb(ctx, s);
+    if ((rc = gb.capture([&](hipStream_t t) -> int {
+           for (int64_t q = 0; q < block; q += 2) {
+             int r1 = sweep<double>(ctx, t, g, rho, v, tmp, false);
+             if (r1) return r1;
+             if ((r1 = sweep<double>(ctx, t, g, rho, tmp, v, false))) return r1;
+           }
+           return QM_OK;
+         })))
+      return rc;
+    while (done + block <= max_sweeps) {
+      // without checks replay all blocks at once
+      const int64_t times = checks ? 1 : (max_sweeps - done) / block;
+      if ((rc = gb.replay(times, (int)block))) return rc;
+      done += times * block;
+      if (checks) {
+        if ((rc = host_delta(ctx, s, QM_F64, v, tmp, n, &delta))) return rc;
+        if (delta <= tol) {
+          converged = true;
+          break;
+        }
+      }
+    }
+  }
+  while (!converged && done + 2 <= max_sweeps) {
     if ((rc = sweep<double>(ctx, s, g, rho, v, tmp))) return rc;
     if ((rc = sweep<double>(ctx, s, g, rho, tmp, v))) return rc;
     done += 2;
-    if (tol >= 0 && check_every > 0 && done % check_every == 0) {
+    if (checks && done % check_every == 0) {
       if ((rc = host_delta(ctx, s, QM_F64, v, tmp, n, &delta))) return rc;
-      if (delta <= tol) {
-        converged = true;
-        break;
-      }
+      if (delta <= tol) converged = true;
     }
   }
   if (!converged && done < max_sweeps) {   // odd count: one more sweep, result copied back
@@ -217,6 +382,14 @@ int jacobi_fp64(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho,
   return QM_OK;
 }
 
+__global__ void k_fill0(float* __restrict__ a, float* __restrict__ b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    a[i] = 0.f;
+    b[i] = 0.f;
+  }
+}
+
 int jacobi_fp32(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho, double* v, int64_t max_sweeps,
                 double tol, int block, int64_t* done_out, double* delta_out) {
   const int64_t n = npoints(g);
@@ -226,33 +399,52 @@ int jacobi_fp32(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho,
   if ((rc = qm_ws(ctx, WS_JAC_B, sizeof(float) * n, (void**)&e))) return rc;
   if ((rc = qm_ws(ctx, WS_JAC_C, sizeof(float) * n, (void**)&t))) return rc;
   const unsigned upd_blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
-  int64_t done = 0;
-  double delta = -1.0;
-  bool converged = false;
-  while (done < max_sweeps) {
-    const int64_t m = std::min<int64_t>(block, max_sweeps - done);
-    hipLaunchKernelGGL(k_residual, sweep_grid(g), dim3(kBX, kBY), 0, s, rho, (const double*)v, r, g->nx, g->ny,
+  // one residual block of m sweeps: residual, clear, m FP32 sweeps, correction (kernels only)
+  auto enqueue = [&](hipStream_t q, int64_t m, bool prof) -> int {
+    hipLaunchKernelGGL(k_residual, sweep_grid(g), dim3(kBX, kBY), 0, q, rho, (const double*)v, r, g->nx, g->ny,
                        g->nz, g->step);
-    QM_CHECK_LAUNCH(ctx);
-    QM_CHECK_HIP(ctx, hipMemsetAsync(e, 0, sizeof(float) * n, s));
-    QM_CHECK_HIP(ctx, hipMemsetAsync(t, 0, sizeof(float) * n, s));
+    hipLaunchKernelGGL(k_fill0, dim3(upd_blocks), dim3(256), 0, q, e, t, n);
     float* a = e;
     float* b = t;
     for (int64_t j = 0; j < m; ++j) {
-      if ((rc = sweep<float>(ctx, s, g, r, a, b))) return rc;
+      int r1 = sweep<float>(ctx, q, g, r, a, b, prof);
+      if (r1) return r1;
       std::swap(a, b);
     }
-    // a = e after m sweeps, b = after m-1 sweeps
-    if (tol >= 0) {
-      if ((rc = host_delta(ctx, s, QM_F32, a, b, n, &delta))) return rc;
+    hipLaunchKernelGGL(k_add_correction, dim3(upd_blocks), dim3(256), 0, q, v, (const float*)a, n);
+    return hipGetLastError() == hipSuccess ? QM_OK : QM_EHIP;
+  };
+  int64_t done = 0;
+  double delta = -1.0;
+  bool converged = false;
+  const bool checks = tol >= 0;
+  if (max_sweeps >= block) {
+    GraphBlock gb(ctx, s);
+    if ((rc = gb.capture([&](hipStream_t q) { return enqueue(q, block, false); }))) return rc;
+    while (done + block <= max_sweeps) {
+      const int64_t times = checks ? 1 : (max_sweeps - done) / block;
+      if ((rc = gb.replay(times, block))) return rc;
+      done += times * block;
+      if (checks) {
+        // after a replay, e holds the last sweep and t the one before (block is even)
+        if ((rc = host_delta(ctx, s, QM_F32, e, t, n, &delta))) return rc;
+        if (delta <= tol) {
+          converged = true;
+          break;
+        }
+      }
     }
-    hipLaunchKernelGGL(k_add_correction, dim3(upd_blocks), dim3(256), 0, s, v, (const float*)a, n);
-    QM_CHECK_LAUNCH(ctx);
-    done += m;
-    if (tol >= 0 && delta <= tol) {
-      converged = true;
-      break;
+  }
+  if (!converged && done < max_sweeps) {
+    const int64_t m = max_sweeps - done;
+    if ((rc = enqueue(s, m, true))) return qm_fail(ctx, rc, "Jacobi (FP32): launch failed");
+    if (checks) {
+      float* last = (m & 1) ? t : e;
+      float* prev = (m & 1) ? e : t;
+      if ((rc = host_delta(ctx, s, QM_F32, last, prev, n, &delta))) return rc;
+      converged = delta <= tol;
     }
+    done = max_sweeps;
   }
   if (done_out) *done_out = done;
   if (delta_out) *delta_out = delta;
@@ -313,6 +505,7 @@ int qm_jacobi(qm_ctx* ctx, void* stream, const qm_geom* g, int precision, const 
   if ((rc = qm_check_geom(ctx, g))) return rc;
   QM_REQUIRE(ctx, rho && v, QM_EINVAL, "jacobi: null argument");
   QM_REQUIRE(ctx, max_sweeps >= 0, QM_EINVAL, "jacobi: negative sweep count");
+  qm_reclaim_graphs(ctx, false);
   hipStream_t s = (hipStream_t)stream;
   if (precision == QM_PREC_FP64) {
     QM_REQUIRE(ctx, tmp != nullptr, QM_EINVAL, "jacobi: FP64 path needs a second buffer");
@@ -321,7 +514,8 @@ int qm_jacobi(qm_ctx* ctx, void* stream, const qm_geom* g, int precision, const 
                        check_every, sweeps_done, last_delta);
   }
   if (precision == QM_PREC_FP32) {
-    QM_REQUIRE(ctx, check_every > 0, QM_EINVAL, "jacobi: FP32 path needs a residual interval");
+    QM_REQUIRE(ctx, check_every > 0 && check_every % 2 == 0, QM_EINVAL,
+               "jacobi: FP32 path needs an even residual interval");
     return jacobi_fp32(ctx, s, g, (const double*)rho, (double*)v, max_sweeps, tolerance, check_every,
                        sweeps_done, last_delta);
   }
@@ -351,7 +545,8 @@ struct JItem {
   const double* rho64;
   double* v64;
   float* r32;
-  int blk0, bx, by;
+  int blk0, bx, by;          // pointwise kernels: 64 x 4 tiles, one z plane per block
+  int blk0s, tbx, tby, zs;   // sweep: 64 x 32 tiles (four 64 x 8 strips), zs planes per block
 };
 
 __device__ __forceinline__ int find_item(const JItem* __restrict__ it, int n, int b) {
@@ -374,28 +569,26 @@ __device__ __forceinline__ bool item_point(const JItem& q, int local, int& i, in
   return i < q.nx && j < q.ny;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBX* kBY) void kb_sweep(const JItem* __restrict__ items, int n, int dir) {
-  const int it = find_item(items, n, blockIdx.x);
-  const JItem& q = items[it];
-  int i, j, k;
-  if (!item_point(q, blockIdx.x - q.blk0, i, j, k)) return;
-  const T* rho = (const T*)q.rho;
-  const T* v = (const T*)(dir ? q.b : q.a);
-  T* vout = (T*)(dir ? q.a : q.b);
-  const int64_t sy = q.nx, sz = (int64_t)q.nx * q.ny;
-  const int64_t idx = i + sy * j + sz * k;
-  if (i == 0 || j == 0 || k == 0 || i == q.nx - 1 || j == q.ny - 1 || k == q.nz - 1) {
-    vout[idx] = v[idx];
-    return;
+__device__ __forceinline__ int find_item_s(const JItem* __restrict__ it, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (it[mid].blk0s <= b) lo = mid;
+    else hi = mid - 1;
   }
-  T s = v[idx - 1] + v[idx + 1];
-  s += v[idx + sy];
-  s += v[idx - sy];
-  s += v[idx + sz];
-  s += v[idx - sz];
-  s -= rho[idx] / (T)q.h;
-  vout[idx] = s / (T)6;
+  return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSW* kSWaves) void kb_sweep(const JItem* __restrict__ items, int n, int dir) {
+  const JItem& q = items[find_item_s(items, n, blockIdx.x)];
+  const int local = blockIdx.x - q.blk0s;
+  const int bx = local % q.tbx;
+  const int rest = local / q.tbx;
+  const int by = rest % q.tby;
+  const int bz = rest / q.tby;
+  strip_body<T>(bx, by, bz, q.zs, (const T*)q.rho, (const T*)(dir ? q.b : q.a), (T*)(dir ? q.a : q.b), q.nx, q.ny,
+                  q.nz, (T)q.h);
 }
 
 __global__ __launch_bounds__(kBX* kBY) void kb_residual(const JItem* __restrict__ items, int n) {
@@ -473,7 +666,7 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
   qm_reclaim_graphs(ctx, false);
   const bool f64 = precision == QM_PREC_FP64;
   std::vector<JItem> items(n);
-  int blocks = 0;
+  int blocks = 0, sblocks = 0;
   size_t f32_elems = 0;
   for (int i = 0; i < n; ++i) {
     if ((rc = qm_check_geom(ctx, &geoms[i]))) return rc;
@@ -486,6 +679,11 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
     q.bx = (q.nx + kBX - 1) / kBX;
     q.by = (q.ny + kBY - 1) / kBY;
     q.blk0 = blocks;
+    q.zs = pick_zs(q.nx, q.ny, q.nz, ctx->jacobi_zs);
+    q.tbx = (q.nx + kSW - 1) / kSW;
+    q.tby = (q.ny + kTileY - 1) / kTileY;
+    q.blk0s = sblocks;
+    sblocks += q.tbx * q.tby * ((q.nz + q.zs - 1) / q.zs);
     const int64_t nb = (int64_t)q.bx * q.by * q.nz;
     QM_REQUIRE(ctx, blocks + nb < 0x7fffffff, QM_ELIMIT, "jacobi_batch: batch too large");
     blocks += (int)nb;
@@ -519,17 +717,18 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
   QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, items.data(), sizeof(JItem) * items.size(), hipMemcpyHostToDevice, s));
   QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // pageable source
   const JItem* dt = (const JItem*)dtab;
-  const dim3 grid(blocks), thr(kBX, kBY);
+  const dim3 grid(blocks), sgrid(sblocks), thr(kBX, kBY), sthr(kSW, kSWaves);
   // one block of work on stream t; kf = sweeps in the block
   auto enqueue_block = [&](hipStream_t t, int64_t kf) -> int {
     if (f64) {
       for (int64_t q = 0; q < kf; ++q) {
-        hipLaunchKernelGGL(kb_sweep<double>, grid, thr, 0, t, dt, n, (int)(q & 1));
+        hipLaunchKernelGGL(kb_sweep<double>, sgrid, sthr, 0, t, dt, n, (int)(q & 1));
       }
       if (kf & 1) hipLaunchKernelGGL(kb_copy_back, grid, thr, 0, t, dt, n);
     } else {
       hipLaunchKernelGGL(kb_residual, grid, thr, 0, t, dt, n);
-      for (int64_t q = 0; q < kf; ++q) hipLaunchKernelGGL(kb_sweep<float>, grid, thr, 0, t, dt, n, (int)(q & 1));
+      for (int64_t q = 0; q < kf; ++q)
+        hipLaunchKernelGGL(kb_sweep<float>, sgrid, sthr, 0, t, dt, n, (int)(q & 1));
       hipLaunchKernelGGL(kb_add, grid, thr, 0, t, dt, n, (int)(kf & 1));
     }
     return hipGetLastError() == hipSuccess ? QM_OK : QM_EHIP;

@@ -65,6 +65,7 @@ enum WsSlot {
   WS_MISC,
   WS_BATCH,        // batched-launch tables
   WS_BATCH2,
+  WS_VNE,          // diffusion V_ne partials and loop state
   WS_COUNT
 };
 
@@ -74,6 +75,7 @@ struct FftPlanCache;
 struct ProfRec {
   int id;
   hipEvent_t a, b;
+  int weight;   // launches covered by the interval (a graph replay covers several)
 };
 
 // an instantiated graph kept alive until the event recorded after its last launch fires
@@ -90,9 +92,11 @@ struct qm_ctx {
   double* host_scalars = nullptr;   // pinned host buffer for results
   FftPlanCache* fft = nullptr;
   bool prof = false;
-  hipStream_t graph_stream = nullptr;
-  std::vector<LiveGraph> graphs;     // graphs whose replays may still run  // non-blocking stream for graph capture (default priority)
-  int split_tm = 128;               // rows per workgroup of the 16-bit split contraction (64 or 128)
+  hipStream_t graph_stream = nullptr;   // non-blocking stream for graph capture (default priority)
+  std::vector<LiveGraph> graphs;        // graphs whose replays may still run
+  int split_tm = 128;                   // rows per workgroup of the 16-bit split contraction (64 or 128)
+  int f64_direct = 0;                   // FP64 contraction variant: 0 LDS-staged, 1 barrier-free
+  int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)
   std::vector<ProfRec> recs;
 };
 
@@ -110,6 +114,7 @@ struct qm_mol {
   bool has_dm = false;
   std::vector<double> h_P;          // nbf x nbf total P_eff (host)
   double* P64 = nullptr;            // [nbf_pad][nbf_pad]
+  double* M64 = nullptr;            // upper triangle of P, off-diagonal doubled (same shape)
   void* P16 = nullptr;              // fp16 hi/lo, K8-interleaved, 2 planes
   void* Pb16 = nullptr;             // bf16 hi/lo, K8-interleaved, 2 planes
   void* gto = nullptr;              // Hermite distributions of the GTO potential (gto.hip)
@@ -164,16 +169,16 @@ void qm_reclaim_graphs(qm_ctx* ctx, bool wait);
 struct ProfScope {
   qm_ctx* ctx;
   hipStream_t s;
-  int id;
+  int id, weight;
   hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(qm_ctx* c, hipStream_t st, int kid) : ctx(c), s(st), id(kid) {
-    if (ctx->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+  ProfScope(qm_ctx* c, hipStream_t st, int kid, int w = 1, bool on = true) : ctx(c), s(st), id(kid), weight(w) {
+    if (on && ctx->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
       (void)hipEventRecord(a, s);
   }
   ~ProfScope() {
     if (a && b) {
       (void)hipEventRecord(b, s);
-      ctx->recs.push_back({id, a, b});
+      ctx->recs.push_back({id, a, b, weight});
     }
   }
 };

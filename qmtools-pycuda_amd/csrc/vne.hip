@@ -4,6 +4,9 @@
 //   V(v) = [sum_{s in ndiv^3} sum_A Z_A / |r_s - R_A|] / (cnt h^3)
 //   r_s = origin + i h + (s + 1/2) h / ndiv,  cnt = #(s, A) with r_s != R_A.
 // One thread per voxel; nuclei are wave-uniform reads.
+#include <algorithm>
+#include <cmath>
+
 #include "qm_internal.h"
 
 namespace {
@@ -63,4 +66,278 @@ extern "C" int qm_vne_classic(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_g
     return qm_fail(ctx, QM_EINVAL, "vne: bad dtype");
   QM_CHECK_LAUNCH(ctx);
   return QM_OK;
+}
+
+// ===========================================================================
+// Diffusion / adsorption pseudo-nuclear field (reference kernel_vne.cu:13-128,
+// loop qmtools.py:2677-2745).  One sweep, for voxel r with value c:
+//   out = sum_{26 neighbours, loop x, y, z = -1..1} DIF V(r+n) / sqrt(|nx|+|ny|+|nz|)
+//       + (1 - DIF * 16.794682450997072) c - c ADS
+//       + sum_A [d_A < sqrt(3) h] Z_A exp(-d_A),  d_A from the voxel centre (r + 1/2) h
+//   out = 0 on the outer voxel layer.
+// The reference wraps neighbour indices periodically; only outer-layer voxels
+// reach a wrapped neighbour and their value is replaced by 0, so reading
+// clamped indices gives the same field.
+//
+// Kernel: a workgroup of 64 x 4 threads owns a 64 x 8 xy tile (2 rows per
+// thread) and marches through zs planes.  Planes k-1, k, k+1 of the tile with
+// its one-voxel halo sit in three of four rotating LDS slots; the loads of
+// plane k+2 are issued before plane k is computed and stored into the fourth
+// slot afterwards, so each plane costs one barrier.  Nuclei within reach of
+// the workgroup's box are listed once per workgroup (ballot compaction keeps
+// the atom order).  The second sweep of a pair also writes per-workgroup
+// partial sums of |c - out|; a one-block kernel adds them in a fixed order
+// and applies the loop's stop rule on the device, so later sweeps that were
+// already queued return at once and the host reads the state only every few
+// dozen pairs.
+namespace {
+
+constexpr double kVneSelf = 16.794682450997072;   // kernel_vne.cu:84
+constexpr int kVX = 64, kVY = 8, kVTY = 4;
+constexpr int kVW = kVX + 2, kVH = kVY + 2, kVPlane = kVW * kVH;
+constexpr int kVLoads = (kVPlane + 255) / 256;
+constexpr int kVMaxNear = 64;
+
+struct VneState {
+  int stop;            // 1: no more sweeps
+  int not_converged;   // stopped by the pair cap
+  long long pairs;     // pairs of sweeps done
+  double delta;        // last sum |res - out| / npts
+};
+
+__device__ __forceinline__ double vne_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_vne_sweep(const T* __restrict__ v, T* __restrict__ out,
+                                                   const double4* __restrict__ atoms, int natoms, GeomView g, int zs,
+                                                   double ads, double dif, double* __restrict__ part,
+                                                   const VneState* __restrict__ st) {
+  if (st->stop) return;   // same value for every thread
+  __shared__ T pl[4][kVPlane];
+  __shared__ int near[kVMaxNear];
+  __shared__ int nnear;
+  __shared__ double red[kVTY];
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + kVX * ty;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int x0 = blockIdx.x * kVX, y0 = blockIdx.y * kVY;
+  const int z0 = blockIdx.z * zs, z1 = min(nz, z0 + zs);
+  const double h = g.h;
+  const double rcut = sqrt(3.0) * h;
+  // nuclei that can reach a voxel centre of this workgroup (conservative box test)
+  if (ty == 0) {
+    const double lx = (x0 + 0.5) * h, hx = (min(x0 + kVX, nx) - 0.5) * h;
+    const double ly = (y0 + 0.5) * h, hy = (min(y0 + kVY, ny) - 0.5) * h;
+    const double lz = (z0 + 0.5) * h, hz = (z1 - 0.5) * h;
+    const double lim = rcut * (1.0 + 1e-9) + 1e-12;
+    int cnt = 0;
+    for (int a0 = 0; a0 < natoms; a0 += 64) {
+      const int a = a0 + tx;
+      bool hit = false;
+      if (a < natoms) {
+        const double4 A = atoms[a];
+        const double ax = A.x - g.ox, ay = A.y - g.oy, az = A.z - g.oz;
+        const double dx = fmax(0.0, fmax(lx - ax, ax - hx));
+        const double dy = fmax(0.0, fmax(ly - ay, ay - hy));
+        const double dz = fmax(0.0, fmax(lz - az, az - hz));
+        hit = dx * dx + dy * dy + dz * dz <= lim * lim;
+      }
+      const unsigned long long m = __ballot(hit);
+      if (hit) {
+        const int pos = cnt + __popcll(m & ((1ull << tx) - 1ull));
+        if (pos < kVMaxNear) near[pos] = a;
+      }
+      cnt += __popcll(m);
+    }
+    if (tx == 0) nnear = cnt;
+  }
+  // per-thread plane offsets of its share of the 66 x 10 halo tile (clamped into the grid)
+  int64_t off[kVLoads];
+  int slot_e[kVLoads];
+#pragma unroll
+  for (int u = 0; u < kVLoads; ++u) {
+    const int e = t + 256 * u;
+    slot_e[u] = e;
+    const int r = min(e, kVPlane - 1) / kVW, c = min(e, kVPlane - 1) % kVW;
+    const int gx = min(max(x0 - 1 + c, 0), nx - 1), gy = min(max(y0 - 1 + r, 0), ny - 1);
+    off[u] = gx + (int64_t)nx * gy;
+  }
+  const int64_t sz = (int64_t)nx * ny;
+  auto plane_at = [&](int k) -> int64_t { return sz * min(max(k, 0), nz - 1); };
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const int64_t pk = plane_at(z0 - 1 + p);
+#pragma unroll
+    for (int u = 0; u < kVLoads; ++u)
+      if (slot_e[u] < kVPlane) pl[(z0 - 1 + p) & 3][slot_e[u]] = v[pk + off[u]];
+  }
+  __syncthreads();
+  const int n_near = nnear;
+  const bool all_atoms = n_near > kVMaxNear;
+  const int nlist = all_atoms ? natoms : n_near;
+  const double w1 = dif, w2 = dif / sqrt(2.0), w3 = dif / sqrt(3.0);
+  const double self = 1.0 - dif * kVneSelf;
+  const int gx = x0 + tx;
+  double acc = 0.0;
+  for (int k = z0; k < z1; ++k) {
+    T nxt[kVLoads];
+    const int64_t pk2 = plane_at(k + 2);
+#pragma unroll
+    for (int u = 0; u < kVLoads; ++u) nxt[u] = slot_e[u] < kVPlane ? v[pk2 + off[u]] : (T)0;
+    const T* P[3] = {pl[(k - 1) & 3], pl[k & 3], pl[(k + 1) & 3]};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ly = 2 * ty + q + 1, gy = y0 + 2 * ty + q;
+      const int lc = ly * kVW + tx + 1;
+      const double c = (double)P[1][lc];
+      double sum = 0.0;
+#pragma unroll
+      for (int x = -1; x <= 1; ++x)
+#pragma unroll
+        for (int y = -1; y <= 1; ++y)
+#pragma unroll
+          for (int z = -1; z <= 1; ++z) {
+            const int d = (x != 0) + (y != 0) + (z != 0);
+            if (d == 0) continue;
+            const double w = d == 1 ? w1 : (d == 2 ? w2 : w3);
+            sum += w * (double)P[z + 1][lc + y * kVW + x];
+          }
+      sum += self * c;
+      sum -= c * ads;
+      const double px = (gx + 0.5) * h, py = (gy + 0.5) * h, pz = (k + 0.5) * h;
+      for (int idx = 0; idx < nlist; ++idx) {
+        const double4 A = atoms[all_atoms ? idx : near[idx]];
+        const double ex = px - (A.x - g.ox), ey = py - (A.y - g.oy), ez = pz - (A.z - g.oz);
+        const double d = sqrt(ex * ex + ey * ey + ez * ez);
+        if (d < rcut) sum += A.w * exp(-d);
+      }
+      if (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1 || k == 0 || k == nz - 1) sum = 0.0;
+      if (gx < nx && gy < ny) {
+        const T o = (T)sum;
+        out[sz * k + (int64_t)nx * gy + gx] = o;
+        acc += fabs(c - (double)o);
+      }
+    }
+    // plane k+2 goes to the slot of plane k-2, last read before the previous barrier
+#pragma unroll
+    for (int u = 0; u < kVLoads; ++u)
+      if (slot_e[u] < kVPlane) pl[(k + 2) & 3][slot_e[u]] = nxt[u];
+    __syncthreads();
+  }
+  if (part) {
+    acc = vne_wave_sum(acc);
+    if (tx == 0) red[ty] = acc;
+    __syncthreads();
+    if (t == 0) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kVTY; ++w) s += red[w];
+      part[blockIdx.x + gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] = s;
+    }
+  }
+}
+
+__global__ void k_vne_init(VneState* st, int stop) {
+  st->stop = stop;
+  st->not_converged = 0;
+  st->pairs = 0;
+  st->delta = 1.0;
+}
+
+// sums the partials of a pair's second sweep and applies the stop rule of qmtools.py:2715-2741
+__global__ __launch_bounds__(256) void k_vne_finish(const double* __restrict__ part, int nb, double npts, double tol,
+                                                    double maxiters, VneState* st) {
+  if (st->stop) return;
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += part[b];
+  s = vne_wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double delta = (red[0] + red[1] + red[2] + red[3]) / npts;
+    st->delta = delta;
+    st->pairs += 1;
+    if ((double)st->pairs > maxiters) {
+      st->stop = 1;
+      st->not_converged = 1;
+    } else if (!(delta > tol)) {
+      st->stop = 1;
+    }
+  }
+}
+
+template <typename T>
+int vne_diffusion(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, double ads, double dif, double tol,
+                  double maxiters, T* v, T* tmp, int64_t* pairs_out, double* delta_out, int* conv_out) {
+  const int64_t n = npoints(g);
+  const unsigned tbx = (unsigned)((g->nx + kVX - 1) / kVX), tby = (unsigned)((g->ny + kVY - 1) / kVY);
+  const int64_t tiles = (int64_t)tbx * tby;
+  const int64_t tz = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+  const int zs = (int)std::max<int64_t>(4, (g->nz + tz - 1) / tz);
+  const unsigned tbz = (unsigned)((g->nz + zs - 1) / zs);
+  const int64_t nb = tiles * tbz;
+  QM_REQUIRE(ctx, nb < (1 << 30), QM_ELIMIT, "vne: grid too large");
+  void* w = nullptr;
+  int rc;
+  if ((rc = qm_ws(ctx, WS_VNE, sizeof(double) * (size_t)(nb + 8), &w))) return rc;
+  VneState* st = (VneState*)w;
+  double* part = (double*)w + 8;
+  QM_CHECK_HIP(ctx, hipMemsetAsync(v, 0, sizeof(T) * n, s));
+  QM_CHECK_HIP(ctx, hipMemsetAsync(tmp, 0, sizeof(T) * n, s));
+  hipLaunchKernelGGL(k_vne_init, dim3(1), dim3(1), 0, s, st, (1.0 > tol) ? 0 : 1);
+  QM_CHECK_LAUNCH(ctx);
+  const GeomView gv = geom_view(g);
+  const dim3 grid(tbx, tby, tbz), block(kVX, kVTY);
+  const double4* atoms = (const double4*)mol->atoms;
+  // pairs before the cap stops the loop: floor(maxiters) + 1
+  const double cap = std::floor(std::max(maxiters, -1.0)) + 1.0;
+  int64_t queued = 0;
+  int64_t chunk = 8;
+  VneState hs{};
+  for (;;) {
+    for (int64_t p = 0; p < chunk; ++p) {
+      ProfScope ps(ctx, s, QM_K_VNE);
+      hipLaunchKernelGGL(k_vne_sweep<T>, grid, block, 0, s, (const T*)v, tmp, atoms, mol->natoms, gv, zs, ads, dif,
+                         (double*)nullptr, (const VneState*)st);
+      hipLaunchKernelGGL(k_vne_sweep<T>, grid, block, 0, s, (const T*)tmp, v, atoms, mol->natoms, gv, zs, ads, dif,
+                         part, (const VneState*)st);
+      hipLaunchKernelGGL(k_vne_finish, dim3(1), dim3(256), 0, s, (const double*)part, (int)nb, (double)n, tol,
+                         maxiters, st);
+    }
+    QM_CHECK_LAUNCH(ctx);
+    queued += chunk;
+    QM_CHECK_HIP(ctx, hipMemcpyAsync(&hs, st, sizeof(VneState), hipMemcpyDeviceToHost, s));
+    QM_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    if (hs.stop || (double)queued >= cap + 1.0) break;
+    chunk = std::min<int64_t>(chunk * 2, 64);
+  }
+  if (pairs_out) *pairs_out = hs.pairs;
+  if (delta_out) *delta_out = hs.delta;
+  if (conv_out) *conv_out = hs.not_converged ? 0 : 1;
+  return QM_OK;
+}
+
+}  // namespace
+
+extern "C" int qm_vne_diffusion(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int dtype, double adsorb,
+                                double diff, double tolerance, double maxiters, void* v, void* tmp,
+                                int64_t* pairs, double* delta, int* converged) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  if ((rc = qm_check_geom(ctx, g))) return rc;
+  QM_REQUIRE(ctx, mol && v && tmp && v != tmp, QM_EINVAL, "vne_diffusion: null or aliased buffers");
+  QM_REQUIRE(ctx, std::isfinite(adsorb) && std::isfinite(diff) && std::isfinite(maxiters) && maxiters < 1e12,
+             QM_EINVAL, "vne_diffusion: adsorb, diff and a finite maxiters required");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QM_F64)
+    return vne_diffusion<double>(ctx, mol, s, g, adsorb, diff, tolerance, maxiters, (double*)v, (double*)tmp, pairs,
+                                 delta, converged);
+  if (dtype == QM_F32)
+    return vne_diffusion<float>(ctx, mol, s, g, adsorb, diff, tolerance, maxiters, (float*)v, (float*)tmp, pairs,
+                                delta, converged);
+  return qm_fail(ctx, QM_EINVAL, "vne_diffusion: bad dtype");
 }

@@ -70,6 +70,7 @@ SIGNATURES = {
     "qm_hartree_gto_points": [_vp, _vp, _vp, _vp, _i64, _vp],
     "qm_hartree_gto_grid": [_vp, _vp, _vp, _pg, _i, _vp],
     "qm_vne_classic": [_vp, _vp, _vp, _pg, _i, _i, _vp],
+    "qm_vne_diffusion": [_vp, _vp, _vp, _pg, _i, _d, _d, _d, _d, _vp, _vp, _c.POINTER(_i64), _pd, _c.POINTER(_i)],
     "qm_density_batch": [_vp, _vp, _i, _c.POINTER(_vp), _pg, _c.POINTER(_vp), _i, _i, _i],
     "qm_jacobi_batch": [_vp, _vp, _i, _pg, _c.POINTER(_vp), _c.POINTER(_vp), _c.POINTER(_vp), _i, _i64, _i, _i],
     "qm_tune_set": [_vp, _i, _i],
@@ -78,8 +79,8 @@ SIGNATURES = {
     "qm_profile": [_vp, _i],
     "qm_profile_read": [_vp, _i, _pd, _c.POINTER(_i64)],
 }
-QM_TUNE_SPLIT_ROWS = 0
-QM_K_DENSITY_CONTRACT, QM_K_DENSITY_SPLIT, QM_K_PHI, QM_K_PHI_SPLIT, QM_K_FFT_AXIS, QM_K_GTO, QM_K_JACOBI = range(7)
+QM_TUNE_SPLIT_ROWS, QM_TUNE_F64_DIRECT, QM_TUNE_JACOBI_ZS = 0, 1, 2
+QM_K_DENSITY_CONTRACT, QM_K_DENSITY_SPLIT, QM_K_PHI, QM_K_PHI_SPLIT, QM_K_FFT_AXIS, QM_K_GTO, QM_K_JACOBI, QM_K_VNE = range(8)
 _RESTYPE = {"qm_last_error": _c.c_char_p}
 
 _lock = threading.Lock()

@@ -244,6 +244,41 @@ class QMTools:
         return qgrid
 
     @staticmethod
+    def Compute_VNe(grid, molecule, adsorb=0.1, diff=0.01, tolerance=1.0e-9, maxiters=None, copyBack=True,
+                    precision=None):
+        """Diffusion / adsorption pseudo-nuclear field (reference `qmtools.py:2677-2745`, `kernel_vne.cu`).
+
+        Pairs of sweeps from 0 until sum|res - out| / npts <= tolerance; after
+        more than `maxiters` pairs (default 10 npts^(1/3)) it stops with a
+        RuntimeWarning, where the reference prints "VNe not converged!".  The
+        grid argument is a template; the result is a new single-field Grid
+        (FP64, or FP32 for the reduced-precision modes).  The pair count and
+        last delta are kept as `.pairs` and `.delta` on the result.
+        """
+        p = _prec(precision)
+        dt = numpy.float64 if p == "fp64" else numpy.float32
+        gres = Grid.emptyAs(grid, nfields=1, dtype=dt)
+        tmp = Grid.emptyAs(grid, nfields=1, dtype=dt)
+        if maxiters is None:
+            maxiters = 10 * numpy.power(grid.npts, 1.0 / 3)
+        dm = device_molecule(molecule)
+        pairs = ctypes.c_int64(0)
+        delta = ctypes.c_double(0.0)
+        conv = ctypes.c_int(1)
+        nat.call("qm_vne_diffusion", dm.handle, nat.stream(), ctypes.byref(nat.geom(gres)),
+                 nat.QM_F64 if dt == numpy.float64 else nat.QM_F32, float(adsorb), float(diff), float(tolerance),
+                 float(maxiters), nat.dptr(gres.d_qube), nat.dptr(tmp.d_qube), ctypes.byref(pairs),
+                 ctypes.byref(delta), ctypes.byref(conv))
+        gres.pairs = pairs.value
+        gres.delta = delta.value
+        if not conv.value:
+            warnings.warn(f"Compute_VNe: not converged after {pairs.value} pairs (delta {delta.value:.3e}, "
+                          f"maxiters {maxiters})", RuntimeWarning)
+        if copyBack:
+            gres.download()
+        return gres
+
+    @staticmethod
     def Compute_VNe_classic(grid, molecule, divisions=4, copyBack=True):
         """Classical nuclear potential averaged over divisions^3 sub-points (reference `qmtools.py:2750-2790`)."""
         gout = Grid.emptyAs(grid, nfields=1, dtype=numpy.float64)

@@ -59,13 +59,18 @@ def _say(*a):
 
 
 # ---------------------------------------------------------------- density
-@pytest.mark.parametrize("subgrid", [1, 2])
-def test_density_fp64_synthetic(qm, small, subgrid):
+@pytest.mark.parametrize("subgrid,direct", [(1, 0), (2, 0), (1, 1), (2, 1)])
+def test_density_fp64_synthetic(qm, small, subgrid, direct):
     from oracle import density as od
+    from qmtools import autotune
     mol, grid, sysm = small
     nx, ny, nz, h, o = grid.geometry()
-    _say("density fp64 synthetic", grid.shape, "subgrid", subgrid, "nbf", mol.nbf)
-    q = qm.QMTools.Compute_density(grid, mol, subgrid=subgrid)
+    _say("density fp64 synthetic", grid.shape, "subgrid", subgrid, "nbf", mol.nbf, "direct", direct)
+    autotune.set("f64_direct", direct)
+    try:
+        q = qm.QMTools.Compute_density(grid, mol, subgrid=subgrid)
+    finally:
+        autotune.set("f64_direct", 0)
     want, raw = od.density_grid(sysm, nx, ny, nz, h, o, subgrid=subgrid)
     assert _rel(q.qube, want) <= 1e-10
     assert q.raw_total == pytest.approx(raw, rel=1e-10)
@@ -272,12 +277,18 @@ def large(request, qm, obasis):
     return mol, grid, _system(mol, obasis)
 
 
-def test_density_fp64_large_basis(qm, large):
+@pytest.mark.parametrize("direct", [0, 1])
+def test_density_fp64_large_basis(qm, large, direct):
     from oracle import density as od
+    from qmtools import autotune
     mol, grid, sysm = large
     nx, ny, nz, h, o = grid.geometry()
-    _say("density fp64 large basis", grid.shape, "nbf", mol.nbf)
-    q = qm.QMTools.Compute_density(grid, mol)
+    _say("density fp64 large basis", grid.shape, "nbf", mol.nbf, "direct", direct)
+    autotune.set("f64_direct", direct)
+    try:
+        q = qm.QMTools.Compute_density(grid, mol)
+    finally:
+        autotune.set("f64_direct", 0)
     want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
     assert _rel(q.qube, want) <= 1e-10
     assert q.raw_total == pytest.approx(raw, rel=1e-10)

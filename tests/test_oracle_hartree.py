@@ -321,3 +321,97 @@ def test_vne_classic_skips_coincident_subpoint():
         if d != 0:
             acc += 3 / d
     assert v[idx] == pytest.approx(acc / 7 / h ** 3, rel=1e-13)
+
+
+# ----------------------------------------------------------------------------- diffusion V_ne
+
+
+def _vne_sweep_bruteforce(v, types, coords, h, origin, ads, dif):
+    """Point-by-point restatement of one sweep with explicit periodic index wrap."""
+    nz, ny, nx = v.shape
+    out = numpy.zeros_like(v)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                s = 0.0
+                for x in (-1, 0, 1):
+                    for y in (-1, 0, 1):
+                        for z in (-1, 0, 1):
+                            d = abs(x) + abs(y) + abs(z)
+                            if d == 0:
+                                continue
+                            s += dif * v[(k + z) % nz, (j + y) % ny, (i + x) % nx] / math.sqrt(d)
+                c = v[k, j, i]
+                s += (1.0 - dif * ov.VNE_SELF) * c
+                s -= c * ads
+                p = ((i + 0.5) * h, (j + 0.5) * h, (k + 0.5) * h)
+                for zt, r in zip(types, coords):
+                    d = math.sqrt(sum((p[a] - (r[a] - origin[a])) ** 2 for a in range(3)))
+                    if d < math.sqrt(3.0) * h:
+                        s += zt * math.exp(-d)
+                if i in (0, nx - 1) or j in (0, ny - 1) or k in (0, nz - 1):
+                    s = 0.0
+                out[k, j, i] = s
+    return out
+
+
+def test_vne_sweep_bruteforce():
+    rng = numpy.random.default_rng(3)
+    v = rng.standard_normal((6, 5, 7))
+    types = [6.0, 1.0]
+    origin = (-0.3, 0.2, 0.1)
+    h = 0.4
+    coords = [(origin[0] + 2.5 * h + 0.05, origin[1] + 1.5 * h, origin[2] + 3.5 * h - 0.1),
+              (origin[0] + 4.2 * h, origin[1] + 2.7 * h, origin[2] + 2.2 * h)]
+    a = ov.vne_sweep(v, types, coords, h, origin, 0.1, 0.03)
+    b = _vne_sweep_bruteforce(v, types, coords, h, origin, 0.1, 0.03)
+    assert numpy.max(numpy.abs(a - b)) < 1e-13
+
+
+def test_vne_constant_field_weights():
+    """A constant interior field gains DIF*(sum of the 26 weights - 16.7947) - ADS per sweep."""
+    v = numpy.full((7, 7, 7), 2.0)
+    out = ov.vne_sweep(v, [], [], 0.3, (0, 0, 0), 0.05, 0.02)
+    wsum = 6 + 12 / math.sqrt(2) + 8 / math.sqrt(3)
+    expect = 2.0 * (1 + 0.02 * (wsum - 16.794682450997072) - 0.05)
+    assert abs(out[3, 3, 3] - expect) < 1e-14
+    assert out[0, 3, 3] == 0.0 and out[3, 6, 3] == 0.0
+
+
+def test_vne_source_geometry_and_cutoff():
+    """Atom on a voxel centre: Z there, Z e^{-h} on face neighbours, Z e^{-sqrt2 h} on edges, 0 on corners."""
+    h = 0.25
+    origin = (1.0, -2.0, 0.5)
+    c = (origin[0] + 3.5 * h, origin[1] + 3.5 * h, origin[2] + 3.5 * h)   # centre of voxel (3,3,3)
+    out = ov.vne_sweep(numpy.zeros((8, 8, 8)), [7.0], [c], h, origin, 0.1, 0.01)
+    assert abs(out[3, 3, 3] - 7.0) < 1e-14
+    assert abs(out[3, 3, 4] - 7.0 * math.exp(-h)) < 1e-14
+    assert abs(out[4, 2, 3] - 7.0 * math.exp(-math.sqrt(2) * h)) < 1e-14
+    assert out[4, 4, 4] == 0.0                     # distance sqrt(3) h is not < sqrt(3) h
+    assert numpy.count_nonzero(out) == 1 + 6 + 12
+
+
+def test_vne_pure_adsorption_geometric_series():
+    """With DIF = 0 every interior voxel follows v <- (1 - a) v + S: v_m = S (1 - (1 - a)^m) / a."""
+    h = 0.3
+    origin = (0.0, 0.0, 0.0)
+    c = (3.5 * h, 3.5 * h, 3.5 * h)
+    a = 0.2
+    res, n, delta, conv = ov.vne_diffusion([5.0], [c], 8, 8, 8, h, origin, adsorb=a, diff=0.0, tolerance=-1.0,
+                                           maxiters=4)
+    assert n == 5 and not conv
+    m = 2 * n
+    assert abs(res[3, 3, 3] - 5.0 * (1 - (1 - a) ** m) / a) < 1e-12
+    s1 = 5.0 * math.exp(-h)
+    assert abs(res[3, 3, 2] - s1 * (1 - (1 - a) ** m) / a) < 1e-12
+    # whole field and the last pair's delta |v_m - v_{m-1}| / npts from the source S of one sweep
+    S = ov.vne_sweep(numpy.zeros((8, 8, 8)), [5.0], [c], h, origin, a, 0.0)
+    assert numpy.max(numpy.abs(res - S * (1 - (1 - a) ** m) / a)) < 1e-12
+    assert abs(delta - float(numpy.sum(S)) * (1 - a) ** (m - 1) / 512) < 1e-14
+
+
+def test_vne_loop_control():
+    res, n, delta, conv = ov.vne_diffusion([1.0], [(0.5, 0.5, 0.5)], 6, 6, 6, 0.2, (0, 0, 0), tolerance=1.0)
+    assert n == 0 and conv and not res.any()
+    res, n, delta, conv = ov.vne_diffusion([1.0], [(0.5, 0.5, 0.5)], 6, 6, 6, 0.2, (0, 0, 0), tolerance=1e-3)
+    assert conv and delta <= 1e-3 and n >= 1

@@ -293,13 +293,16 @@ int vne_diffusion(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, dou
   const GeomView gv = geom_view(g);
   const dim3 grid(tbx, tby, tbz), block(kVX, kVTY);
   const double4* atoms = (const double4*)mol->atoms;
-  // pairs before the cap stops the loop: floor(maxiters) + 1
-  const double cap = std::floor(std::max(maxiters, -1.0)) + 1.0;
+  // the loop runs at least one pair and stops once the count exceeds maxiters:
+  // at most max(1, floor(maxiters) + 1) pairs, and nothing is queued beyond that
+  const int64_t cap = std::max<int64_t>(1, (int64_t)std::floor(std::max(maxiters, -1.0)) + 1);
   int64_t queued = 0;
   int64_t chunk = 8;
   VneState hs{};
   for (;;) {
-    for (int64_t p = 0; p < chunk; ++p) {
+    const int64_t m = std::min(chunk, cap - queued);
+    if (m <= 0) break;
+    for (int64_t p = 0; p < m; ++p) {
       ProfScope ps(ctx, s, QM_K_VNE);
       hipLaunchKernelGGL(k_vne_sweep<T>, grid, block, 0, s, (const T*)v, tmp, atoms, mol->natoms, gv, zs, ads, dif,
                          (double*)nullptr, (const VneState*)st);
@@ -309,12 +312,13 @@ int vne_diffusion(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, dou
                          maxiters, st);
     }
     QM_CHECK_LAUNCH(ctx);
-    queued += chunk;
+    queued += m;
     QM_CHECK_HIP(ctx, hipMemcpyAsync(&hs, st, sizeof(VneState), hipMemcpyDeviceToHost, s));
     QM_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    if (hs.stop || (double)queued >= cap + 1.0) break;
+    if (hs.stop) break;
     chunk = std::min<int64_t>(chunk * 2, 64);
   }
+
   if (pairs_out) *pairs_out = hs.pairs;
   if (delta_out) *delta_out = hs.delta;
   if (conv_out) *conv_out = hs.not_converged ? 0 : 1;

@@ -117,3 +117,17 @@ def test_cache_ingest_round_trip(qm, tmp_path):
     b = cache.load_batch(str(tmp_path), basis, step=0.4, fat=2.0)[0]
     assert numpy.array_equal(b["qgrid"].qube, e["qgrid"].qube)
     assert numpy.array_equal(b["gvne"].qube, e["gvne"].qube)
+
+
+def test_vne_diffusion_negative_cap_runs_one_pair(qm):
+    from qmtools import synth
+    basis = qm.BasisSet(BASIS)
+    mol = synth.synthetic_molecule(4, 2, basis)
+    grid = synth.centred_grid(mol, 16, step_bohr=0.4, pad_ang=2.0)
+    _say("vne diffusion maxiters=-0.5", grid.shape)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        v = qm.QMTools.Compute_VNe(grid, mol, tolerance=-1.0, maxiters=-0.5)
+    want, pairs, _, conv = _oracle(mol, grid, tolerance=-1.0, maxiters=-0.5)
+    assert v.pairs == pairs == 1 and not conv
+    assert _rel(v.qube, want) <= 1e-12

@@ -415,3 +415,10 @@ def test_vne_loop_control():
     assert n == 0 and conv and not res.any()
     res, n, delta, conv = ov.vne_diffusion([1.0], [(0.5, 0.5, 0.5)], 6, 6, 6, 0.2, (0, 0, 0), tolerance=1e-3)
     assert conv and delta <= 1e-3 and n >= 1
+
+
+def test_vne_loop_runs_one_pair_below_zero_cap():
+    """The stop test follows the pair, so a negative cap still runs one pair (qmtools.py:2715-2741)."""
+    res, n, delta, conv = ov.vne_diffusion([1.0], [(0.5, 0.5, 0.5)], 6, 6, 6, 0.2, (0, 0, 0), tolerance=-1.0,
+                                           maxiters=-0.5)
+    assert n == 1 and not conv and res.any()

@@ -36,7 +36,11 @@ import numpy  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per contraction launch measured with rocprofv3 FETCH_SIZE (x2 calibration) and
 # WRITE_SIZE (x1) for this workload; source: profiles/config2_pmc_traffic.md
-MEASURED_TRAFFIC = {}
+# (precision, nbf_pad, variant): (points per launch, HBM bytes per launch)
+MEASURED_TRAFFIC = {("fp64", 640, "lds"): (838784, 4442023 * 1024 * 2 + 190037 * 1024),
+                    ("fp64", 640, "direct"): (838784, 7899833 * 1024 * 2 + 6553 * 1024),
+                    ("mixed", 640, 64): (1677696, 3792138 * 1024 * 2 + 530834 * 1024),
+                    ("mixed", 640, 128): (1677824, 6706858 * 1024 * 2 + 9129267 * 1024)}
 FP64_MFMA_PEAK_TFLOPS = 78.6
 F16_MFMA_PEAK_TFLOPS = 2500.0
 
@@ -271,8 +275,9 @@ def synthetic_batch(basis, count, atoms, grid_n, steps, seed):
 
 
 def batched_section(basis, count, sweeps, seed=41):
-    """Config 4 (reduced count): mixed batched fused launches + hipGraph Jacobi vs the plain FP64 path
-    (one molecule at a time: FP64 density, FP64 Jacobi), same Poisson method and sweep count."""
+    """Config 4 (reduced count): mixed batched fused launches vs the plain FP64 path (one molecule at a
+    time through the drop-in API), for each Poisson method: Jacobi (hipGraph, `sweeps` sweeps, multipole
+    boundary) and free-space FFT.  Same method and sweep count on both sides of each ratio."""
     import torch
     import qmtools
     from qmtools import batch
@@ -280,29 +285,34 @@ def batched_section(basis, count, sweeps, seed=41):
     npts = batch.grid_points(grids)
     log(f"batched: {count} molecules, {npts} grid points, {sweeps} Jacobi sweeps")
     lane = batch.Lane(default=True)
-    batch.run(mols[:2], grids[:2], "mixed", 100, lane=lane)       # warm-up (workspaces, tuning)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    qs, vs = batch.run(mols, grids, "mixed", sweeps, lane=lane)
-    torch.cuda.synchronize()
-    t_mixed = time.perf_counter() - t0
-    # plain FP64 path, one molecule at a time through the drop-in API
-    t0 = time.perf_counter()
-    plain = []
-    for m, g in zip(mols, grids):
-        q = qmtools.QMTools.Compute_density(g, m, precision="fp64", copyBack=False)
-        v = qmtools.QMTools.Compute_hartree_iter(q, m, tolerance=None, max_sweeps=sweeps, boundary="multipole",
-                                                 copyBack=False)
-        plain.append((q.d_qube, v.d_qube))
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    rho_err = max(float((q.double() - p[0]).abs().max() / p[0].abs().max()) for q, p in zip(qs, plain))
-    chg_err = max(float(abs(q.double().sum() - p[0].sum()) / p[0].sum()) for q, p in zip(qs, plain))
-    v_err = max(float((v - p[1]).abs().max() / p[1].abs().max()) for v, p in zip(vs, plain))
-    return {"molecules": count, "grid_points": npts, "sweeps": sweeps,
-            "mixed_batched_pts_per_s": npts / t_mixed, "plain_fp64_pts_per_s": npts / t_plain,
-            "speedup": t_plain / t_mixed, "seconds": {"mixed_batched": t_mixed, "plain_fp64": t_plain},
-            "max_rel_err_vs_fp64": {"rho": rho_err, "charge": chg_err, "v_hartree": v_err}}
+    out = {"molecules": count, "grid_points": npts, "sweeps": sweeps}
+    for poisson in ("jacobi", "fft"):
+        batch.run(mols[:2], grids[:2], "mixed", 100, lane=lane, poisson=poisson)     # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        qs, vs = batch.run(mols, grids, "mixed", sweeps, lane=lane, poisson=poisson)
+        torch.cuda.synchronize()
+        t_mixed = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        plain = []
+        for m, g in zip(mols, grids):
+            q = qmtools.QMTools.Compute_density(g, m, precision="fp64", copyBack=False)
+            if poisson == "jacobi":
+                v = qmtools.QMTools.Compute_hartree_iter(q, m, tolerance=None, max_sweeps=sweeps,
+                                                         boundary="multipole", copyBack=False).d_qube
+            else:
+                v = batch.hartree_fft_batch([m], [g], [q.d_qube], "fp64", lane=lane)[0]
+            plain.append((q.d_qube, v))
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        rho_err = max(float((q.double() - p[0]).abs().max() / p[0].abs().max()) for q, p in zip(qs, plain))
+        chg_err = max(float(abs(q.double().sum() - p[0].sum()) / p[0].sum()) for q, p in zip(qs, plain))
+        v_err = max(float((v.double() - p[1]).abs().max() / p[1].abs().max()) for v, p in zip(vs, plain))
+        out[poisson] = {"mixed_batched_pts_per_s": npts / t_mixed, "plain_fp64_pts_per_s": npts / t_plain,
+                        "speedup": t_plain / t_mixed, "seconds": {"mixed_batched": t_mixed, "plain_fp64": t_plain},
+                        "max_rel_err_vs_fp64": {"rho": rho_err, "charge": chg_err, "v_hartree": v_err}}
+        del qs, vs, plain
+    return out
 
 
 def served_section(basis, nreq, rate, sweeps, seed=77):
@@ -336,6 +346,7 @@ def main():
     ap.add_argument("--serve", type=int, default=48, help="requests in the served section (0: skip)")
     ap.add_argument("--rate", type=float, default=100.0, help="Poisson arrival rate of the served section")
     ap.add_argument("--sweeps", type=int, default=2000, help="Jacobi sweeps in the batched/served sections")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 Jacobi and diffusion V_ne sections")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
@@ -415,6 +426,9 @@ def main():
     achieved = flops_per_launch * mult / (avg_ms * 1e-3) / 1e12
     dense_equiv = (2.0 * nbf * nbf + 2.0 * nbf) * launch_pts * mult / (avg_ms * 1e-3) / 1e12
     peak = FP64_MFMA_PEAK_TFLOPS if args.precision == "fp64" else F16_MFMA_PEAK_TFLOPS
+    variant = ("direct" if dense else "lds") if args.precision == "fp64" else tune["mixed"].get("split_rows")
+    meas = MEASURED_TRAFFIC.get((args.precision, (nbf + 63) // 64 * 64, variant))
+    traffic = None if meas is None else meas[1] / meas[0] * launch_pts
 
     check = None
     if not args.no_check and args.precision == "fp64":
@@ -427,14 +441,17 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(w)
 
-    jac = jacobi_section(w)
-    log(f"jacobi: {jac}")
-    vne = vne_section(w)
-    log(f"vne diffusion: {vne}")
+    jac = vne = None
+    if not args.no_extra:
+        jac = jacobi_section(w)
+        log(f"jacobi: {jac}")
+        vne = vne_section(w)
+        log(f"vne diffusion: {vne}")
     batched = served = None
     if args.batch > 0:
         batched = batched_section(w.basis, args.batch, args.sweeps)
-        log(f"batched: {batched['speedup']:.2f}x plain FP64, errors {batched['max_rel_err_vs_fp64']}")
+        for k in ("jacobi", "fft"):
+            log(f"batched ({k}): {batched[k]['speedup']:.2f}x plain FP64, errors {batched[k]['max_rel_err_vs_fp64']}")
     if args.serve > 0:
         served = served_section(w.basis, args.serve, args.rate, args.sweeps)
         log(f"served: {served}")
@@ -461,8 +478,7 @@ def main():
                    "nbf": nbf, "points": w.npoints},
         "roofline": {"bound": "mfma", "kernel": "density contraction (Phi P Phi^T diagonal)",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": MEASURED_TRAFFIC.get((args.precision, (nbf + 63) // 64 * 64,
-                                                      int(round(launch_pts)))),
+                     "traffic": traffic,
                      "traffic_source": "profiles/config2_pmc_traffic.md",
                      "algorithmic": (f"({'2 nbf^2' if dense else 'nbf (nbf + 1)'} + 2 nbf) flops/point x "
                                      f"{launch_pts:.0f} points/launch"

@@ -514,6 +514,20 @@ __device__ __forceinline__ void contract_split_body(int blk, const float* __rest
       stage_store(0, pre);
     }
     __syncthreads();
+    // M operands of a stage: unconditional loads (a tile past the matrix or past
+    // its last stage reads element 0, unused), one stage ahead of their use
+    V8 bh[NT], bl[NT];
+    auto load_b = [&](int st, V8* h, V8* l) {
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        const int n = tile[ct] * 16 + li;
+        const bool live = n < nbf_pad && 2 * st <= tile[ct] && st < nks_pass;
+        const int64_t o = live ? ((int64_t)(st * KG + lk) * nbf_pad + n) * 8 : 0;
+        h[ct] = *(const V8*)(p_hi + o);
+        l[ct] = *(const V8*)(p_lo + o);
+      }
+    };
+    load_b(0, bh, bl);
     for (int s = 0; s < nks_pass; ++s) {
       const bool more = s + 1 < nks_pass;
       if (more && stager) {
@@ -521,16 +535,8 @@ __device__ __forceinline__ void contract_split_body(int blk, const float* __rest
 #pragma unroll
         for (int q = 0; q < 8; ++q) pre[q] = nsrc[(int64_t)q * ld];
       }
-      // unconditional loads: a tile past the matrix or past its last stage reads element 0 (unused)
-      V8 bh[NT], bl[NT];
-#pragma unroll
-      for (int ct = 0; ct < NT; ++ct) {
-        const int n = tile[ct] * 16 + li;
-        const bool live = n < nbf_pad && 2 * s <= tile[ct];
-        const int64_t o = live ? ((int64_t)(s * KG + lk) * nbf_pad + n) * 8 : 0;
-        bh[ct] = *(const V8*)(p_hi + o);
-        bl[ct] = *(const V8*)(p_lo + o);
-      }
+      V8 nh[NT], nl[NT];
+      load_b(s + 1, nh, nl);
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) {
         const uint4 uh = Ah[s & 1][lk][rt * 16 + li];
@@ -547,6 +553,11 @@ __device__ __forceinline__ void contract_split_body(int blk, const float* __rest
         }
       }
       if (more && stager) stage_store((s + 1) & 1, pre);
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        bh[ct] = nh[ct];
+        bl[ct] = nl[ct];
+      }
       __syncthreads();
     }
     {

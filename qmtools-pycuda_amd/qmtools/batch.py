@@ -110,11 +110,49 @@ def hartree_jacobi_batch(mols, grids, densities, sweeps=2000, precision="mixed",
     return vs
 
 
-def run(mols, grids, precision="mixed", sweeps=2000, boundary="multipole", lane=None, use_graph=True):
-    """Density + Jacobi Hartree for a batch; returns (densities, potentials) as device tensors."""
+def hartree_fft_batch(mols, grids, densities, precision="mixed", mode="freespace", sigma=0.2, cutoff=6.0,
+                      lane=None):
+    """FFT Poisson potential of electrons + Gaussian nuclei for every molecule: volts, flat device arrays
+    in the C order V[i, j, k] that `QMTools.Compute_hartree_fft` returns (reference qmtools.py:2637-2639).
+
+    The transforms run one molecule after the other on the lane's stream; the
+    context caches the twiddle tables and free-space Green's-function transforms
+    per grid shape, so molecules on the same grid reuse them.
+    """
+    lane = lane or Lane(default=True)
+    t = _device.torch()
+    dev = _device.require()
+    f64 = precision == "fp64"
+    dt = t.float64 if f64 else t.float32
+    m = {"periodic": nat.QM_FFT_PERIODIC, "freespace": nat.QM_FFT_FREESPACE}[mode]
+    outs = []
+    with t.cuda.stream(lane.stream):
+        for mol, g, q in zip(mols, grids, densities):
+            q = q.to(dt)
+            out = t.empty(int(g.npts), dtype=dt, device=dev)
+            nat.call_ctx(lane.ctx, "qm_hartree_fft", device_molecule(mol).handle, lane.handle,
+                         ctypes.byref(nat.geom(g)), nat.QM_PREC_FP64 if f64 else nat.QM_PREC_FP32,
+                         nat.QM_F64 if f64 else nat.QM_F32, nat.dptr(q), float(sigma), float(cutoff), m,
+                         nat.dptr(out))
+            outs.append(out)
+    return outs
+
+
+def run(mols, grids, precision="mixed", sweeps=2000, boundary="multipole", lane=None, use_graph=True,
+        poisson="jacobi"):
+    """Density + Hartree for a batch; returns (densities, potentials) as device tensors.
+
+    poisson="jacobi": V = -phi/(4 pi) after `sweeps` sweeps (FP64 arrays);
+    poisson="fft": free-space FFT potential in volts (FP64 or FP32 arrays, C order V[i, j, k]).
+    """
     lane = lane or Lane(default=True)
     qs = density_batch(mols, grids, precision, lane)
-    vs = hartree_jacobi_batch(mols, grids, qs, sweeps, precision, boundary, lane, use_graph)
+    if poisson == "fft":
+        vs = hartree_fft_batch(mols, grids, qs, precision, lane=lane)
+    elif poisson == "jacobi":
+        vs = hartree_jacobi_batch(mols, grids, qs, sweeps, precision, boundary, lane, use_graph)
+    else:
+        raise ValueError("poisson must be 'jacobi' or 'fft'")
     return qs, vs
 
 

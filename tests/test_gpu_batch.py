@@ -37,6 +37,28 @@ def test_density_batch_equals_single(items, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fft_batch_equals_single(items, precision):
+    """Batched FFT Poisson equals the drop-in call molecule by molecule (FP64 bitwise; FP32 <= 1e-6 vs FP64)."""
+    import qmtools
+    from qmtools import batch
+    mols = [m for m, _ in items]
+    grids = [g for _, g in items]
+    _say("fft batch", precision, [g.shape for g in grids])
+    qs, vs = batch.run(mols, grids, precision, poisson="fft")
+    for (m, g), q, v in zip(items, qs, vs):
+        qg = qmtools.Grid.emptyAs(g)
+        qg.d_qube = q.double().clone()
+        want = qmtools.QMTools.Compute_hartree_fft(qg, m, mode="freespace", precision=precision)
+        got = v.double().cpu().numpy()
+        got = got.reshape(g.space_shape())           # both are V[i, j, k] in C order
+        if precision == "fp64":
+            numpy.testing.assert_array_equal(got, want)
+        else:
+            ref = qmtools.QMTools.Compute_hartree_fft(qg, m, mode="freespace", precision="fp64")
+            assert numpy.abs(got - ref).max() <= 1e-6 * numpy.abs(ref).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_jacobi_batch_equals_single(items, precision):
     import qmtools
     from qmtools import batch

@@ -418,7 +418,7 @@ def main():
     # the barrier-free FP64 variant is dense.
     nbf = w.mol.nbf
     launch_pts = w.N / max(k_n / args.steps, 1)
-    dense = args.precision == "fp64" and tune["fp64"].get("f64_direct", 0) == 1
+    dense = args.precision == "fp64" and tune["fp64"].get("f64_variant", 0) == 1
     per_point = (2.0 * nbf * nbf if dense else nbf * (nbf + 1.0)) + 2.0 * nbf
     flops_per_launch = per_point * launch_pts
     mult = 1.0 if args.precision == "fp64" else 3.0     # hi/lo split executes 3 products
@@ -426,7 +426,8 @@ def main():
     achieved = flops_per_launch * mult / (avg_ms * 1e-3) / 1e12
     dense_equiv = (2.0 * nbf * nbf + 2.0 * nbf) * launch_pts * mult / (avg_ms * 1e-3) / 1e12
     peak = FP64_MFMA_PEAK_TFLOPS if args.precision == "fp64" else F16_MFMA_PEAK_TFLOPS
-    variant = ("direct" if dense else "lds") if args.precision == "fp64" else tune["mixed"].get("split_rows")
+    variant = ({0: "lds", 1: "direct", 2: "tiles"}[tune["fp64"].get("f64_variant", 0)] if args.precision == "fp64"
+               else tune["mixed"].get("split_variant"))
     meas = MEASURED_TRAFFIC.get((args.precision, (nbf + 63) // 64 * 64, variant))
     traffic = None if meas is None else meas[1] / meas[0] * launch_pts
 

@@ -245,8 +245,10 @@ int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* geoms, cons
 
 /* Tile-shape knobs of the density contraction and the Jacobi sweep (no reference counterpart). */
 enum {
-  QM_TUNE_SPLIT_ROWS = 0,   /* points per workgroup of the 16-bit split contraction: 64 or 128 */
-  QM_TUNE_F64_DIRECT = 1,   /* FP64 contraction: 0 = basis tile staged in LDS, 1 = per-wave loads, no barriers */
+  QM_TUNE_SPLIT_VARIANT = 0, /* 16-bit split contraction: 0 = 128 x 128 tiles with per-block partial sums,
+                               64 or 128 = points per workgroup of the all-columns kernel */
+  QM_TUNE_F64_VARIANT = 1,  /* FP64 contraction: 0 = all columns per workgroup (wave-cyclic tiles), 1 = per-wave
+                               loads of the dense P, no barriers, 2 = 128 x 128 tiles with per-block partial sums */
   QM_TUNE_JACOBI_ZS = 2,    /* z planes per Jacobi sweep workgroup, 0 = chosen per grid, else 1..4096 */
   QM_TUNE_COUNT = 3
 };

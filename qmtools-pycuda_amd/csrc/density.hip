@@ -460,6 +460,253 @@ struct Vec8<__bf16> {
 // MFMA 16x16x32: A lane l = (t: l%16, k: 8(l/16)+0..7), B likewise, D lane l, reg r = (t: 4(l/16)+r, n: l%16).
 // Staging: thread (row, k-group) loads 8 values of one row (each load instruction covers 64
 // consecutive rows, 256 B) and writes one 16-byte hi and lo word to LDS.
+// ---------------------------------------------------------------------------
+// Tiled FP64 contraction (variant 2).  A workgroup of 8 waves owns a tile of
+// 128 points x 128 columns (column block cb) and forms X = Phi M over
+// k < min(nbf_pad, 128 (cb + 1)) only (M = D + 2U is upper triangular); waves
+// are arranged 4 x 2 and own 32 x 64 sub-tiles (8 f64 16x16x4 MFMA tiles).
+// Both operands go through double-buffered LDS in stages of 16 k; the global
+// loads of stage s+1 are issued before stage s is computed.  The workgroup
+// writes the partial row-dot sum_{n in block} Phi_tn X_tn of its 128 points to
+// part[cb][t]; k_tiles_finish adds the blocks in a fixed order.  Small
+// accumulators and 72 KB of LDS let two workgroups share a CU.
+// Workgroups are numbered so that the column blocks of one row tile are
+// consecutive on the same XCD (workgroup b runs on XCD b mod 8) and share its
+// L2 copy of the row tile's basis values; the heaviest block goes first.
+constexpr int kTT = 128, kTKS = 16, kTPad = 8, kTWG = 512;
+
+__device__ __forceinline__ bool tile_coords(int b, int nrt, int ncb, int& rt, int& cb) {
+  const int xcd = b & 7, slot = b >> 3;
+  const int rl = slot / ncb;
+  cb = ncb - 1 - (slot - rl * ncb);
+  rt = rl * 8 + xcd;
+  return rt < nrt;
+}
+
+inline int tile_grid(int nrt, int ncb) { return (nrt + 7) / 8 * 8 * ncb; }
+
+__device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __restrict__ phi, int ld,
+                                               const double* __restrict__ M, int np, double* __restrict__ part,
+                                               int ldp) {
+  __shared__ double As[2][kTKS][kTT + kTPad];
+  __shared__ double Bs[2][kTKS][kTT + kTPad];
+  __shared__ double red[2][kTT];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wr = wave >> 1, wc = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int row0 = rt * kTT, c0 = cb * kTT;
+  const int nst = min(np, c0 + kTT) / kTKS;
+  constexpr int Q = kTKS * kTT / kTWG;          // 4 values of each operand per thread and stage
+  constexpr int KQ = kTWG / kTT;                // k rows covered by one q step
+  const int kk0 = t / kTT, ii = t % kTT;
+  // per-thread offsets; the per-stage and per-q parts of the addresses are uniform
+  const unsigned aoff = (unsigned)(kk0 * ld + ii);
+  const unsigned boff = (unsigned)(kk0 * np + min(c0 + ii, np - 1));   // columns past the matrix are never used
+  const double* abase = phi + row0;
+  double pa[Q], pb[Q];
+  auto load = [&](int st) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = st * kTKS + KQ * q;
+      pa[q] = (abase + (int64_t)k * ld)[aoff];
+      pb[q] = (M + (int64_t)k * np)[boff];
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      As[buf][kk0 + KQ * q][ii] = pa[q];
+      Bs[buf][kk0 + KQ * q][ii] = pb[q];
+    }
+  };
+  d4 acc[2][4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = d4{0, 0, 0, 0};
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const bool more = st + 1 < nst;
+    if (more) load(st + 1);
+    const int buf = st & 1;
+#pragma unroll
+    for (int ks = 0; ks < kTKS; ks += 4) {
+      double a[2], b[4];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) a[r] = As[buf][ks + lk][wr * 32 + r * 16 + li];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b[c] = Bs[buf][ks + lk][wc * 64 + c * 16 + li];
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+    }
+    if (more) store((st + 1) & 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    double p4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int n = c0 + wc * 64 + c * 16 + li;
+      if (n < np) {
+        const double* ph = phi + (int64_t)n * ld + row0 + wr * 32 + r * 16 + lk;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p4[q] += acc[r][c][q] * ph[4 * q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double v = sum16(p4[q]);
+      if (li == 0) red[wc][wr * 32 + r * 16 + 4 * q + lk] = v;
+    }
+  }
+  __syncthreads();
+  if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
+}
+
+// Tiled 16-bit split contraction: the tiling, triangle, workgroup order and
+// partial sums of the FP64 tiled kernel, with stages of 32 k.  The basis values
+// are scaled per point and rounded to hi + lo while they are staged; the M
+// planes are copied as stored (K8-interleaved).  X = Ah Bh + Ah Bl + Al Bh in
+// FP32, row-dot with the FP32 basis values in FP64.
+template <typename T16>
+__device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __restrict__ phi, int ld,
+                                                 const float* __restrict__ pscale, const T16* __restrict__ p_hi,
+                                                 const T16* __restrict__ p_lo, int np, double* __restrict__ part,
+                                                 int ldp) {
+  typedef typename Vec8<T16>::type V8;
+  __shared__ uint4 Ah[2][4][kTT], Al[2][4][kTT], Bh[2][4][kTT], Bl[2][4][kTT];
+  __shared__ double red[2][kTT];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wr = wave >> 1, wc = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int row0 = rt * kTT, c0 = cb * kTT;
+  const int nst = min(np, c0 + kTT) / 32;
+  // staging: thread -> row / column ii of the tile and k group g (8 k values of a stage)
+  const int ii = t % kTT, g = t / kTT;
+  const float sc = pscale[row0 + ii];
+  const float* asrc = phi + (int64_t)(8 * g) * ld + row0 + ii;
+  const int64_t bcol = min(c0 + ii, np - 1);   // columns past the matrix are never used
+  float pa[8];
+  uint4 pbh, pbl;
+  auto load = [&](int st) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pa[q] = asrc[(int64_t)(st * 32 + q) * ld];
+    const int64_t o = ((int64_t)(st * 4 + g) * np + bcol) * 8;
+    pbh = *(const uint4*)(p_hi + o);
+    pbl = *(const uint4*)(p_lo + o);
+  };
+  auto store = [&](int buf) {
+    V8 h, l;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float x = pa[q] * sc;
+      const T16 hq = (T16)x;
+      h[q] = hq;
+      l[q] = (T16)(x - (float)hq);
+    }
+    Ah[buf][g][ii] = *(const uint4*)&h;
+    Al[buf][g][ii] = *(const uint4*)&l;
+    Bh[buf][g][ii] = pbh;
+    Bl[buf][g][ii] = pbl;
+  };
+  f4 acc[2][4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = f4{0, 0, 0, 0};
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const bool more = st + 1 < nst;
+    if (more) load(st + 1);
+    const int buf = st & 1;
+    V8 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint4 uh = Ah[buf][lk][wr * 32 + r * 16 + li], ul = Al[buf][lk][wr * 32 + r * 16 + li];
+      ah[r] = *(const V8*)&uh;
+      al[r] = *(const V8*)&ul;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint4 uh = Bh[buf][lk][wc * 64 + c * 16 + li], ul = Bl[buf][lk][wc * 64 + c * 16 + li];
+      bh[c] = *(const V8*)&uh;
+      bl[c] = *(const V8*)&ul;
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        acc[r][c] = Vec8<T16>::mma(al[r], bh[c], acc[r][c]);
+        acc[r][c] = Vec8<T16>::mma(ah[r], bl[c], acc[r][c]);
+        acc[r][c] = Vec8<T16>::mma(ah[r], bh[c], acc[r][c]);
+      }
+    if (more) store((st + 1) & 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    double p4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int n = c0 + wc * 64 + c * 16 + li;
+      if (n < np) {
+        const float4 v = *(const float4*)(phi + (int64_t)n * ld + row0 + wr * 32 + r * 16 + 4 * lk);
+        p4[0] += (double)(acc[r][c][0] * v.x);
+        p4[1] += (double)(acc[r][c][1] * v.y);
+        p4[2] += (double)(acc[r][c][2] * v.z);
+        p4[3] += (double)(acc[r][c][3] * v.w);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double v = sum16(p4[q]);
+      if (li == 0) red[wc][wr * 32 + r * 16 + 4 * lk + q] = v;
+    }
+  }
+  __syncthreads();
+  if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
+}
+
+template <typename T16>
+__global__ __launch_bounds__(kTWG) void k_contract_split_tiles(const float* __restrict__ phi, int ld,
+                                                               const float* __restrict__ pscale,
+                                                               const T16* __restrict__ p_hi,
+                                                               const T16* __restrict__ p_lo, int np, int nrt, int ncb,
+                                                               double* __restrict__ part) {
+  int rt, cb;
+  if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
+  tiled_split_body<T16>(rt, cb, phi, ld, pscale, p_hi, p_lo, np, part, ld);
+}
+
+__global__ __launch_bounds__(kTWG) void k_contract_f64_tiles(const double* __restrict__ phi, int ld,
+                                                             const double* __restrict__ M, int np, int nrt, int ncb,
+                                                             double* __restrict__ part) {
+  int rt, cb;
+  if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
+  tiled_f64_body(rt, cb, phi, ld, M, np, part, ld);
+}
+
+// rho_t = scale * sum_cb part[cb][t] (fixed order), optionally / pscale_t
+template <typename TO>
+__global__ void k_tiles_finish(const double* __restrict__ part, int ldp, int ncb, int npts, double scale,
+                               const float* __restrict__ pscale, TO* __restrict__ out, int64_t v0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  double v = 0.0;
+  for (int cb = 0; cb < ncb; ++cb) v += part[(int64_t)cb * ldp + t];
+  if (pscale) v /= (double)pscale[t];
+  store_rho<TO>(out, v0 + t, v * scale);
+}
+
 template <typename T16, int NT, int TM, typename TO>
 __device__ __forceinline__ void contract_split_body(int blk, const float* __restrict__ phi, int ld,
                                                     const float* __restrict__ pscale,
@@ -748,6 +995,10 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
     rc = qm_ws(ctx, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
     if (rc) return rc;
   }
+  const bool tiled = ctx->f64_variant == 2;
+  const int ncb = (np + kTT - 1) / kTT;
+  double* part = nullptr;
+  if (tiled && (rc = qm_ws(ctx, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
   const GeomView gv = geom_view(g);
   const MolView mv = mol->view();
   const double h3 = g->step * g->step * g->step;
@@ -762,12 +1013,27 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
     }
     QM_CHECK_LAUNCH(ctx);
     ProfScope pc(ctx, s, QM_K_DENSITY_CONTRACT);
-    if (nsub3 == 1) {
+    if (tiled) {
+      const int nrt = ld / kTT;
+      hipLaunchKernelGGL(k_contract_f64_tiles, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const double*)phi, ld,
+                         (const double*)mol->M64, np, nrt, ncb, part);
+      QM_CHECK_LAUNCH(ctx);
+      if (nsub3 == 1) {
+        hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncb,
+                           npts, h3, (const float*)nullptr, out, v0);
+      } else {
+        hipLaunchKernelGGL(k_tiles_finish<double>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld,
+                           ncb, npts, h3 / nsub3, (const float*)nullptr, rho, (int64_t)0);
+        QM_CHECK_LAUNCH(ctx);
+        hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
+                           nsub3, out, v0);
+      }
+    } else if (nsub3 == 1) {
       launch_contract_f64<TO>(s, ld, (const double*)phi, (const double*)mol->P64, (const double*)mol->M64, np, npts, h3, out, v0,
-                              ctx->f64_direct);
+                              ctx->f64_variant);
     } else {
       launch_contract_f64<double>(s, ld, (const double*)phi, (const double*)mol->P64, (const double*)mol->M64, np, npts, h3 / nsub3, rho,
-                                  (int64_t)0, ctx->f64_direct);
+                                  (int64_t)0, ctx->f64_variant);
       QM_CHECK_LAUNCH(ctx);
       hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
                          nsub3, out, v0);
@@ -801,6 +1067,10 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
   }
   const T16* Phi = (const T16*)(std::is_same<T16, _Float16>::value ? mol->P16 : mol->Pb16);
   const T16* Plo = Phi + (int64_t)np * np;
+  const bool tiled = ctx->split_tm == 0;
+  const int ncb = (np + kTT - 1) / kTT;
+  double* part = nullptr;
+  if (tiled && (rc = qm_ws(ctx, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
   const GeomView gv = geom_view(g);
   const MolView mv = mol->view();
   const double h3 = g->step * g->step * g->step / mol->pfac;
@@ -815,7 +1085,22 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     }
     QM_CHECK_LAUNCH(ctx);
     ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
-    if (nsub3 == 1) {
+    if (tiled) {
+      const int nrt = ld / kTT;
+      hipLaunchKernelGGL(k_contract_split_tiles<T16>, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const float*)phi,
+                         ld, (const float*)pscale, Phi, Plo, np, nrt, ncb, part);
+      QM_CHECK_LAUNCH(ctx);
+      if (nsub3 == 1) {
+        hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncb,
+                           npts, h3, (const float*)pscale, out, v0);
+      } else {
+        hipLaunchKernelGGL(k_tiles_finish<double>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld,
+                           ncb, npts, h3 / nsub3, (const float*)pscale, rho, (int64_t)0);
+        QM_CHECK_LAUNCH(ctx);
+        hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
+                           nsub3, out, v0);
+      }
+    } else if (nsub3 == 1) {
       launch_contract_split<T16, TO>(s, ld, (const float*)phi, pscale, Phi, Plo, np, npts, h3, out, v0,
                                      ctx->split_tm);
     } else {
@@ -903,14 +1188,15 @@ int qm_basis_points(qm_ctx* ctx, qm_mol* mol, void* stream, const double* pts, i
 int qm_tune_set(qm_ctx* ctx, int knob, int value) {
   int rc = qm_enter(ctx);
   if (rc) return rc;
-  if (knob == QM_TUNE_SPLIT_ROWS) {
-    QM_REQUIRE(ctx, value == 64 || value == 128, QM_EINVAL, "tune: split rows must be 64 or 128");
+  if (knob == QM_TUNE_SPLIT_VARIANT) {
+    QM_REQUIRE(ctx, value == 0 || value == 64 || value == 128, QM_EINVAL,
+               "tune: split variant must be 0 (tiles), 64 or 128 (rows per workgroup)");
     ctx->split_tm = value;
     return QM_OK;
   }
-  if (knob == QM_TUNE_F64_DIRECT) {
-    QM_REQUIRE(ctx, value == 0 || value == 1, QM_EINVAL, "tune: f64 variant must be 0 or 1");
-    ctx->f64_direct = value;
+  if (knob == QM_TUNE_F64_VARIANT) {
+    QM_REQUIRE(ctx, value >= 0 && value <= 2, QM_EINVAL, "tune: f64 variant must be 0, 1 or 2");
+    ctx->f64_variant = value;
     return QM_OK;
   }
   if (knob == QM_TUNE_JACOBI_ZS) {
@@ -925,12 +1211,12 @@ int qm_tune_get(qm_ctx* ctx, int knob, int* value) {
   int rc = qm_enter(ctx);
   if (rc) return rc;
   QM_REQUIRE(ctx, value != nullptr, QM_EINVAL, "tune: null output");
-  if (knob == QM_TUNE_SPLIT_ROWS) {
+  if (knob == QM_TUNE_SPLIT_VARIANT) {
     *value = ctx->split_tm;
     return QM_OK;
   }
-  if (knob == QM_TUNE_F64_DIRECT) {
-    *value = ctx->f64_direct;
+  if (knob == QM_TUNE_F64_VARIANT) {
+    *value = ctx->f64_variant;
     return QM_OK;
   }
   if (knob == QM_TUNE_JACOBI_ZS) {
@@ -953,8 +1239,8 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   void* out = nullptr;
   if ((rc = qm_ws(ctx, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
-  std::vector<int> cands = split ? std::vector<int>{64, 128} : std::vector<int>{0, 1};
-  int& knob = split ? ctx->split_tm : ctx->f64_direct;
+  std::vector<int> cands = split ? std::vector<int>{0, 64, 128} : std::vector<int>{0, 1, 2};
+  int& knob = split ? ctx->split_tm : ctx->f64_variant;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));
   QM_CHECK_HIP(ctx, hipEventCreate(&b));
@@ -1019,7 +1305,9 @@ struct BChunk {
   const void* pl;       // 16-bit lo plane
   void* out;
   double scale;
-  int blk0_phi;         // first block in the round's basis-value launch
+  double* part;         // tiled kernels: per-column-block partial sums [ncb][ld]
+  int ncb;
+  int blk0_phi;         // first block in the round's basis-value (and tiled finish) launch
   int blk0;             // first block in its contraction launch
 };
 
@@ -1110,6 +1398,34 @@ void launch_batch_split(hipStream_t s, int nt, int tm, const BChunk* c, int n, i
 #undef QM_BSPLIT
 }
 
+__global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __restrict__ c, int n) {
+  const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
+  int rt, cb;
+  if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
+  tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
+}
+
+template <typename T16>
+__global__ __launch_bounds__(kTWG) void kb_contract_split_tiles(const BChunk* __restrict__ c, int n) {
+  const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
+  int rt, cb;
+  if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
+  tiled_split_body<T16>(rt, cb, (const float*)q.phi, q.ld, q.pscale, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad,
+                        q.part, q.ld);
+}
+
+// same sums as k_tiles_finish; one thread per point, blocks numbered like the basis-value launch
+template <typename TO>
+__global__ __launch_bounds__(kPhiThreads) void kb_tiles_finish(const BChunk* __restrict__ c, int n, int split) {
+  const BChunk& q = c[find_chunk(c, n, blockIdx.x, true)];
+  const int t = (blockIdx.x - q.blk0_phi) * kPhiThreads + threadIdx.x;
+  if (t >= q.nvox) return;
+  double v = 0.0;
+  for (int cb = 0; cb < q.ncb; ++cb) v += q.part[(int64_t)cb * q.ld + t];
+  if (split) v /= (double)q.pscale[t];
+  store_rho<TO>((TO*)q.out, q.v0 + t, v * q.scale);
+}
+
 struct Round {
   size_t first, count;                      // chunk range in the table (sorted by NT within the round)
   std::vector<std::pair<int, std::pair<size_t, size_t>>> groups;   // (nt, [first, count))
@@ -1136,11 +1452,12 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   const bool f64 = precision == QM_PREC_FP64;
   const size_t elt = f64 ? sizeof(double) : sizeof(float);
   const int tm = f64 ? 64 : ctx->split_tm;
+  const bool tiled = f64 ? ctx->f64_variant == 2 : ctx->split_tm == 0;
   // chunks per molecule, packed into rounds of at most kPhiBudget bytes of basis values
   std::vector<BChunk> table;
   std::vector<Round> rounds;
   std::vector<BChunk> cur;
-  size_t cur_bytes = 0, cur_ps = 0;
+  size_t cur_bytes = 0, cur_ps = 0, cur_part = 0;
   auto flush = [&]() {
     if (cur.empty()) return;
     Round r;
@@ -1159,7 +1476,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       int blocks = 0;
       while (e < cur.size() && cur[e].nt == cur[k].nt) {
         cur[e].blk0 = blocks;
-        blocks += cur[e].ld / tm;
+        blocks += tiled ? tile_grid(cur[e].ld / kTT, cur[e].ncb) : cur[e].ld / tm;   // tiled: multiples of 8
         ++e;
       }
       r.groups.push_back({cur[k].nt, {r.first + k, e - k}});
@@ -1171,6 +1488,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
     cur.clear();
     cur_bytes = 0;
     cur_ps = 0;
+    cur_part = 0;
   };
   for (int i = 0; i < nmol; ++i) {
     qm_mol* m = mols[i];
@@ -1187,16 +1505,19 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       c.nvox = (int)std::min<int64_t>(cap, N - v0);
       c.ld = (int)round_up(c.nvox, kTileM);
       c.nbf_pad = np;
-      c.nt = pick_nt(np);
+      c.nt = tiled ? 0 : pick_nt(np);     // tiled chunks form one launch per round
+      c.ncb = (np + kTT - 1) / kTT;
       const size_t bytes = (size_t)np * c.ld * elt;
       if (cur_bytes + bytes > kPhiBudget) flush();
       c.phi = (void*)(uintptr_t)cur_bytes;            // offsets for now, rebased below
       c.pscale = (float*)(uintptr_t)cur_ps;
+      c.part = (double*)(uintptr_t)cur_part;
       cur_bytes += bytes;
       cur_ps += (size_t)c.ld * sizeof(float);
+      if (tiled) cur_part += sizeof(double) * (size_t)c.ncb * c.ld;
       const double h3 = geoms[i].step * geoms[i].step * geoms[i].step;
       if (f64) {
-        c.ph = ctx->f64_direct ? m->P64 : m->M64;   // the LDS-staged kernel takes the triangular form
+        c.ph = ctx->f64_variant == 1 ? m->P64 : m->M64;   // the LDS-staged kernel takes the triangular form
         c.pl = nullptr;
         c.scale = h3;
       } else {
@@ -1212,24 +1533,28 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   }
   flush();
   // workspaces: basis values of one round, point scales, and the chunk table
-  size_t max_bytes = 0, max_ps = 0;
+  size_t max_bytes = 0, max_ps = 0, max_part = 0;
   for (const Round& r : rounds) {
-    size_t b = 0, p = 0;
+    size_t b = 0, p = 0, q = 0;
     for (size_t k = 0; k < r.count; ++k) {
       const BChunk& c = table[r.first + k];
       b = std::max(b, (size_t)(uintptr_t)c.phi + (size_t)c.nbf_pad * c.ld * elt);
       p = std::max(p, (size_t)(uintptr_t)c.pscale + (size_t)c.ld * sizeof(float));
+      if (tiled) q = std::max(q, (size_t)(uintptr_t)c.part + sizeof(double) * (size_t)c.ncb * c.ld);
     }
     max_bytes = std::max(max_bytes, b);
     max_ps = std::max(max_ps, p);
+    max_part = std::max(max_part, q);
   }
-  void *phi = nullptr, *ps = nullptr, *dtab = nullptr;
+  void *phi = nullptr, *ps = nullptr, *dtab = nullptr, *pt = nullptr;
   if ((rc = qm_ws(ctx, WS_PHI, max_bytes, &phi))) return rc;
   if ((rc = qm_ws(ctx, WS_PHI2, std::max<size_t>(max_ps, 16), &ps))) return rc;
+  if (tiled && (rc = qm_ws(ctx, WS_PART, max_part, &pt))) return rc;
   if ((rc = qm_ws(ctx, WS_BATCH, sizeof(BChunk) * table.size(), &dtab))) return rc;
   for (BChunk& c : table) {
     c.phi = (char*)phi + (size_t)(uintptr_t)c.phi;
     c.pscale = (float*)((char*)ps + (size_t)(uintptr_t)c.pscale);
+    c.part = tiled ? (double*)((char*)pt + (size_t)(uintptr_t)c.part) : nullptr;
   }
   QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, table.data(), sizeof(BChunk) * table.size(), hipMemcpyHostToDevice, s));
   QM_CHECK_HIP(ctx, hipStreamSynchronize(s));    // the host table is pageable and goes out of scope
@@ -1249,9 +1574,24 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       const int gn = (int)r.groups[gi].second.second;
       const int blocks = r.group_blocks[gi];
       ProfScope pc(ctx, s, f64 ? QM_K_DENSITY_CONTRACT : QM_K_DENSITY_SPLIT);
-      if (f64) {
-        if (out_dtype == QM_F64) launch_batch_f64<double>(s, nt, gc, gn, blocks, ctx->f64_direct);
-        else launch_batch_f64<float>(s, nt, gc, gn, blocks, ctx->f64_direct);
+      if (tiled) {
+        if (f64)
+          hipLaunchKernelGGL(kb_contract_f64_tiles, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+        else if (precision == QM_PREC_F16X3)
+          hipLaunchKernelGGL(kb_contract_split_tiles<_Float16>, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+        else
+          hipLaunchKernelGGL(kb_contract_split_tiles<__bf16>, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+        QM_CHECK_LAUNCH(ctx);
+        // finish over the whole round: chunk blocks numbered as in the basis-value launch
+        if (out_dtype == QM_F64)
+          hipLaunchKernelGGL(kb_tiles_finish<double>, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first,
+                             (int)r.count, f64 ? 0 : 1);
+        else
+          hipLaunchKernelGGL(kb_tiles_finish<float>, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first,
+                             (int)r.count, f64 ? 0 : 1);
+      } else if (f64) {
+        if (out_dtype == QM_F64) launch_batch_f64<double>(s, nt, gc, gn, blocks, ctx->f64_variant == 1);
+        else launch_batch_f64<float>(s, nt, gc, gn, blocks, ctx->f64_variant == 1);
       } else if (precision == QM_PREC_F16X3) {
         if (out_dtype == QM_F64) launch_batch_split<_Float16, double>(s, nt, tm, gc, gn, blocks);
         else launch_batch_split<_Float16, float>(s, nt, tm, gc, gn, blocks);

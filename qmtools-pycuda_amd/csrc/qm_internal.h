@@ -66,6 +66,7 @@ enum WsSlot {
   WS_BATCH,        // batched-launch tables
   WS_BATCH2,
   WS_VNE,          // diffusion V_ne partials and loop state
+  WS_PART,         // per-column-block partial densities of the tiled contraction
   WS_COUNT
 };
 
@@ -94,8 +95,8 @@ struct qm_ctx {
   bool prof = false;
   hipStream_t graph_stream = nullptr;   // non-blocking stream for graph capture (default priority)
   std::vector<LiveGraph> graphs;        // graphs whose replays may still run
-  int split_tm = 128;                   // rows per workgroup of the 16-bit split contraction (64 or 128)
-  int f64_direct = 0;                   // FP64 contraction variant: 0 LDS-staged, 1 barrier-free
+  int split_tm = 0;                     // split contraction: 0 tiled, else rows per workgroup (64 or 128)
+  int f64_variant = 2;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled
   int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)
   std::vector<ProfRec> recs;
 };

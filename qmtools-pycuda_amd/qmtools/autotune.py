@@ -15,7 +15,7 @@ import os
 from . import _device
 from . import _native as nat
 
-_KNOBS = {"split_rows": nat.QM_TUNE_SPLIT_ROWS, "f64_direct": nat.QM_TUNE_F64_DIRECT,
+_KNOBS = {"split_variant": nat.QM_TUNE_SPLIT_VARIANT, "f64_variant": nat.QM_TUNE_F64_VARIANT,
           "jacobi_zs": nat.QM_TUNE_JACOBI_ZS}
 _PREC = {"fp64": nat.QM_PREC_FP64, "mixed": nat.QM_PREC_F16X3, "f16": nat.QM_PREC_F16X3,
          "bf16": nat.QM_PREC_BF16X3}
@@ -74,7 +74,7 @@ def autotune_density(molecule, grid, precision="mixed", use_cache=True, path=Non
     best = ctypes.c_double(0.0)
     nat.call("qm_autotune_density", dm.handle, nat.stream(), ctypes.byref(nat.geom(grid)), _PREC[precision],
              ctypes.byref(best))
-    relevant = ["f64_direct"] if precision == "fp64" else ["split_rows"]
+    relevant = ["f64_variant"] if precision == "fp64" else ["split_variant"]
     entry = {k: get(k) for k in relevant}
     entry["ms"] = best.value
     if use_cache:

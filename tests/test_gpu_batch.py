@@ -23,17 +23,23 @@ def items():
     return out
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_density_batch_equals_single(items, precision):
+@pytest.mark.parametrize("precision,knob,variant", [("fp64", "f64_variant", 2), ("fp64", "f64_variant", 0),
+                                                    ("mixed", "split_variant", 0), ("mixed", "split_variant", 128)])
+def test_density_batch_equals_single(items, precision, knob, variant):
     import qmtools
-    from qmtools import batch
+    from qmtools import autotune, batch
     mols = [m for m, _ in items]
     grids = [g for _, g in items]
-    _say("density batch", precision, [g.shape for g in grids])
-    qs = batch.density_batch(mols, grids, precision)
-    for (m, g), q in zip(items, qs):
-        single = qmtools.QMTools.Compute_density(g, m, precision=precision)
-        numpy.testing.assert_array_equal(q.cpu().numpy(), single.qube.ravel())
+    _say("density batch", precision, knob, variant, [g.shape for g in grids])
+    old = autotune.get(knob)
+    autotune.set(knob, variant)
+    try:
+        qs = batch.density_batch(mols, grids, precision)
+        for (m, g), q in zip(items, qs):
+            single = qmtools.QMTools.Compute_density(g, m, precision=precision)
+            numpy.testing.assert_array_equal(q.cpu().numpy(), single.qube.ravel())
+    finally:
+        autotune.set(knob, old)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

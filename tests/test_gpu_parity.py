@@ -59,18 +59,19 @@ def _say(*a):
 
 
 # ---------------------------------------------------------------- density
-@pytest.mark.parametrize("subgrid,direct", [(1, 0), (2, 0), (1, 1), (2, 1)])
-def test_density_fp64_synthetic(qm, small, subgrid, direct):
+@pytest.mark.parametrize("subgrid,variant", [(1, 0), (2, 0), (1, 1), (2, 1), (1, 2), (2, 2)])
+def test_density_fp64_synthetic(qm, small, subgrid, variant):
     from oracle import density as od
     from qmtools import autotune
     mol, grid, sysm = small
     nx, ny, nz, h, o = grid.geometry()
-    _say("density fp64 synthetic", grid.shape, "subgrid", subgrid, "nbf", mol.nbf, "direct", direct)
-    autotune.set("f64_direct", direct)
+    _say("density fp64 synthetic", grid.shape, "subgrid", subgrid, "nbf", mol.nbf, "variant", variant)
+    old = autotune.get("f64_variant")
+    autotune.set("f64_variant", variant)
     try:
         q = qm.QMTools.Compute_density(grid, mol, subgrid=subgrid)
     finally:
-        autotune.set("f64_direct", 0)
+        autotune.set("f64_variant", old)
     want, raw = od.density_grid(sysm, nx, ny, nz, h, o, subgrid=subgrid)
     assert _rel(q.qube, want) <= 1e-10
     assert q.raw_total == pytest.approx(raw, rel=1e-10)
@@ -98,6 +99,28 @@ def test_density_mixed(qm, small, prec, tol):
     want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
     err = _rel(q.qube, want)
     print(f"{prec}: normwise rho error {err:.3e}, raw total rel err {abs(q.raw_total - raw) / raw:.3e}")
+    assert err <= tol
+    assert abs(q.raw_total - raw) / raw <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f16", 2e-6), ("bf16", 1e-4)])
+@pytest.mark.parametrize("subgrid", [1, 2])
+def test_density_mixed_tiled(qm, small, prec, tol, subgrid):
+    """The 128 x 128 tiled split contraction (partial sums per column block) against the oracle."""
+    from oracle import density as od
+    from qmtools import autotune
+    mol, grid, sysm = small
+    nx, ny, nz, h, o = grid.geometry()
+    _say("density tiled", prec, grid.shape, "subgrid", subgrid)
+    old = autotune.get("split_variant")
+    autotune.set("split_variant", 0)
+    try:
+        q = qm.QMTools.Compute_density(grid, mol, subgrid=subgrid, precision=prec)
+    finally:
+        autotune.set("split_variant", old)
+    want, raw = od.density_grid(sysm, nx, ny, nz, h, o, subgrid=subgrid)
+    err = _rel(q.qube, want)
+    print(f"{prec} tiled subgrid {subgrid}: normwise rho error {err:.3e}")
     assert err <= tol
     assert abs(q.raw_total - raw) / raw <= tol
 
@@ -277,35 +300,37 @@ def large(request, qm, obasis):
     return mol, grid, _system(mol, obasis)
 
 
-@pytest.mark.parametrize("direct", [0, 1])
-def test_density_fp64_large_basis(qm, large, direct):
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_density_fp64_large_basis(qm, large, variant):
     from oracle import density as od
     from qmtools import autotune
     mol, grid, sysm = large
     nx, ny, nz, h, o = grid.geometry()
-    _say("density fp64 large basis", grid.shape, "nbf", mol.nbf, "direct", direct)
-    autotune.set("f64_direct", direct)
+    _say("density fp64 large basis", grid.shape, "nbf", mol.nbf, "variant", variant)
+    old = autotune.get("f64_variant")
+    autotune.set("f64_variant", variant)
     try:
         q = qm.QMTools.Compute_density(grid, mol)
     finally:
-        autotune.set("f64_direct", 0)
+        autotune.set("f64_variant", old)
     want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
     assert _rel(q.qube, want) <= 1e-10
     assert q.raw_total == pytest.approx(raw, rel=1e-10)
 
 
-@pytest.mark.parametrize("rows", [64, 128])
+@pytest.mark.parametrize("rows", [0, 64, 128])
 def test_density_f16_large_basis_tiles(qm, large, rows):
     from oracle import density as od
     from qmtools import autotune
     mol, grid, sysm = large
     nx, ny, nz, h, o = grid.geometry()
     _say("density f16 large basis", grid.shape, "nbf", mol.nbf, "rows", rows)
-    autotune.set("split_rows", rows)
+    old = autotune.get("split_variant")
+    autotune.set("split_variant", rows)
     try:
         q = qm.QMTools.Compute_density(grid, mol, precision="f16")
     finally:
-        autotune.set("split_rows", 128)
+        autotune.set("split_variant", old)
     want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
     err = _rel(q.qube, want)
     print(f"f16 rows {rows} nbf {mol.nbf}: normwise error {err:.3e}")
@@ -319,14 +344,15 @@ def test_autotune_cache(qm, small, tmp_path):
     mol, grid, _ = small
     path = str(tmp_path / "tune.json")
     _say("autotune", grid.shape)
+    old = autotune.get("split_variant")
     e = autotune.autotune_density(mol, grid, "mixed", path=path)
-    assert e["split_rows"] in (64, 128) and e["ms"] > 0
+    assert e["split_variant"] in (0, 64, 128) and e["ms"] > 0
     table = json.load(open(path))
     assert autotune.key_for(mol, "mixed") in table
-    autotune.set("split_rows", 64)
+    autotune.set("split_variant", 64)
     again = autotune.autotune_density(mol, grid, "mixed", path=path)   # served from the cache
-    assert again == e and autotune.get("split_rows") == e["split_rows"]
-    autotune.set("split_rows", 128)
+    assert again == e and autotune.get("split_variant") == e["split_variant"]
+    autotune.set("split_variant", old)
 
 
 def test_seed_shared_nodes_deterministic(qm):

@@ -315,10 +315,10 @@ def batched_section(basis, count, sweeps, seed=41):
     return out
 
 
-def served_section(basis, nreq, rate, sweeps, seed=77):
-    """Config 5 (reduced count): Poisson arrivals into the continuous-batching server (mixed path)."""
+def served_section(basis, nreq, rate, sweeps, seed=77, grid_n=(64, 96, 128)):
+    """Config 5: Poisson arrivals at `rate` req/s into the continuous-batching server (mixed path)."""
     from qmtools import serve
-    mols, grids = synthetic_batch(basis, nreq, (5, 120), (64, 96, 128), (0.1, 0.15), seed)
+    mols, grids = synthetic_batch(basis, nreq, (5, 120), grid_n, (0.1, 0.15), seed)
     srv = serve.Server(nlanes=2, precision="mixed", max_batch=8, sweeps=sweeps, hbm_budget=96 << 30,
                        to_host=False).start()
     srv.submit(mols[0], grids[0]).wait(600)                        # warm-up
@@ -344,7 +344,10 @@ def main():
     ap.add_argument("--no-other", action="store_true", help="skip the comparison run of the other precision")
     ap.add_argument("--batch", type=int, default=16, help="molecules in the batched section (0: skip)")
     ap.add_argument("--serve", type=int, default=48, help="requests in the served section (0: skip)")
-    ap.add_argument("--rate", type=float, default=100.0, help="Poisson arrival rate of the served section")
+    ap.add_argument("--rate", type=str, default="100",
+                    help="Poisson arrival rate(s) of the served section, comma-separated (req/s)")
+    ap.add_argument("--serve-grids", type=str, default="64,96,128",
+                    help="cubic grid sizes drawn for served requests, comma-separated")
     ap.add_argument("--sweeps", type=int, default=2000, help="Jacobi sweeps in the batched/served sections")
     ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 Jacobi and diffusion V_ne sections")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -454,8 +457,14 @@ def main():
         for k in ("jacobi", "fft"):
             log(f"batched ({k}): {batched[k]['speedup']:.2f}x plain FP64, errors {batched[k]['max_rel_err_vs_fp64']}")
     if args.serve > 0:
-        served = served_section(w.basis, args.serve, args.rate, args.sweeps)
-        log(f"served: {served}")
+        gn = tuple(int(x) for x in args.serve_grids.split(","))
+        served = []
+        for rate in (float(x) for x in args.rate.split(",")):
+            st = served_section(w.basis, args.serve, rate, args.sweeps, grid_n=gn)
+            log(f"served: {st}")
+            served.append(st)
+        if len(served) == 1:
+            served = served[0]
 
     value = w.N / (ms_step / 1000.0)
     out = {

@@ -127,6 +127,155 @@ __global__ __launch_bounds__(kSW* kSWaves) void k_sweep_strip(const T* __restric
   strip_body<T>(blockIdx.x, blockIdx.y, blockIdx.z, zs, rho, v, vout, nx, ny, nz, h);
 }
 
+// Two sweeps per pass (V -> V''): the same strips with a second level.  Lane L
+// holds column bx W - 1 + L (W = 62 output columns per strip) and rows
+// j0-2 .. j0+R+1 of V; the first sweep is formed on rows j0-1 .. j0+R at all
+// 64 lanes (lanes 0 and 63 take the column beyond the strip from an extra
+// load), the second on rows j0 .. j0+R-1 at lanes 1 .. 62 from the first-sweep
+// values of its neighbours.  In z the second sweep runs one plane behind the
+// first.  Each point's arithmetic, summation order included, is that of two
+// single sweeps, so the result is bitwise the same.  Per output point and pass
+// V is read (R+4)/R x 64/62 times, rho (R+2)/R x 64/62 times, V'' written once.
+template <int R>
+struct Strip2 {
+  static constexpr int W = kSW - 2;                 // output columns per strip
+  static constexpr int TY = R * kSWaves;            // output rows per workgroup
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void strip2_body(int bx, int by, int bz, int zs, const T* __restrict__ rho,
+                                            const T* __restrict__ v, T* __restrict__ vout, int nx, int ny, int nz,
+                                            T h) {
+  constexpr int R0 = R + 4, R1 = R + 2;
+  const int lane = threadIdx.x;
+  const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int i = bx * Strip2<R>::W - 1 + lane;
+  const int j0 = by * Strip2<R>::TY + wy * R;
+  if (j0 >= ny) return;                    // whole wave idle; the kernel has no barrier
+  const int z0 = bz * zs, z1 = min(nz, z0 + zs);
+  const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+  // clamped indices: values read at them only feed outer-layer points (copied) or points outside the grid
+  const unsigned ic = (unsigned)min(max(i, 0), nx - 1);
+  const unsigned ihc = (unsigned)min(max(lane == 0 ? i - 1 : i + 1, 0), nx - 1);
+  auto row = [&](int j) -> size_t { return sy * (size_t)min(max(j, 0), ny - 1); };
+  auto pln = [&](int k) -> size_t { return sz * (size_t)min(max(k, 0), nz - 1); };
+  const bool edge = lane == 0 || lane == kSW - 1;
+  const bool xb = i <= 0 || i >= nx - 1;            // outer layer (or outside) in x
+  T am[R0], ac[R0], ap[R0];         // V on planes m-1, m, m+1, rows j0-2+q
+  T bmm[R1], bm[R1];                // first sweep on planes m-2, m-1, rows j0-1+p
+  T hx[R1], rc[R1], rm[R1];         // halo column of plane m; rho of planes m and m-1 (rows j0-1+p)
+  const int m0 = z0 - 1;
+#pragma unroll
+  for (int q = 0; q < R0; ++q) {
+    am[q] = v[pln(m0 - 1) + row(j0 - 2 + q) + ic];
+    ac[q] = v[pln(m0) + row(j0 - 2 + q) + ic];
+    ap[q] = v[pln(m0 + 1) + row(j0 - 2 + q) + ic];
+  }
+#pragma unroll
+  for (int p = 0; p < R1; ++p) {
+    rc[p] = rho[pln(m0) + row(j0 - 1 + p) + ic];
+    rm[p] = (T)0;
+    hx[p] = (T)0;
+    bmm[p] = (T)0;
+    bm[p] = (T)0;
+  }
+  if (edge) {
+#pragma unroll
+    for (int p = 0; p < R1; ++p) hx[p] = v[pln(m0) + row(j0 - 1 + p) + ihc];
+  }
+  for (int m = m0; m <= z1; ++m) {
+    // prefetch: V plane m+2, halo and rho of plane m+1
+    T an[R0], hn[R1], rn[R1];
+    const size_t pm2 = pln(m + 2), pm1 = pln(m + 1);
+#pragma unroll
+    for (int q = 0; q < R0; ++q) an[q] = v[pm2 + row(j0 - 2 + q) + ic];
+#pragma unroll
+    for (int p = 0; p < R1; ++p) {
+      rn[p] = rho[pm1 + row(j0 - 1 + p) + ic];
+      hn[p] = (T)0;
+    }
+    if (edge) {
+#pragma unroll
+      for (int p = 0; p < R1; ++p) hn[p] = v[pm1 + row(j0 - 1 + p) + ihc];
+    }
+    // first sweep on plane m
+    const bool zb1 = m <= 0 || m >= nz - 1;
+    T bc[R1];
+#pragma unroll
+    for (int p = 0; p < R1; ++p) {
+      const int j = j0 - 1 + p;
+      const T c = ac[p + 1];
+      T xm = __shfl_up(c, 1, kSW);
+      T xp = __shfl_down(c, 1, kSW);
+      if (lane == 0) xm = hx[p];
+      if (lane == kSW - 1) xp = hx[p];
+      T sum = xm + xp;
+      sum += ac[p + 2];
+      sum += ac[p];
+      sum += ap[p + 1];
+      sum += am[p + 1];
+      sum -= rc[p] / h;
+      const bool outer = zb1 || xb || j <= 0 || j >= ny - 1;
+      bc[p] = outer ? c : sum / (T)6;
+    }
+    // second sweep on plane m-1
+    const int k = m - 1;
+    if (k >= z0) {
+      const bool zb2 = k == 0 || k == nz - 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        const T c = bm[r + 1];
+        const T xm = __shfl_up(c, 1, kSW);
+        const T xp = __shfl_down(c, 1, kSW);
+        T sum = xm + xp;
+        sum += bm[r + 2];
+        sum += bm[r];
+        sum += bc[r + 1];
+        sum += bmm[r + 1];
+        sum -= rm[r + 1] / h;
+        const bool outer = zb2 || xb || j == 0 || j == ny - 1;
+        const T val = outer ? c : sum / (T)6;
+        if (lane >= 1 && lane <= kSW - 2 && i < nx && j < ny) vout[sz * (size_t)k + sy * (size_t)j + (unsigned)i] = val;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      am[q] = ac[q];
+      ac[q] = ap[q];
+      ap[q] = an[q];
+    }
+#pragma unroll
+    for (int p = 0; p < R1; ++p) {
+      bmm[p] = bm[p];
+      bm[p] = bc[p];
+      rm[p] = rc[p];
+      rc[p] = rn[p];
+      hx[p] = hn[p];
+    }
+  }
+}
+
+constexpr int kR2D = 4, kR2F = 8;   // rows per strip of the two-sweep kernel: FP64, FP32
+template <typename T>
+constexpr int r2() { return sizeof(T) == 8 ? kR2D : kR2F; }
+
+template <typename T>
+__global__ __launch_bounds__(kSW* kSWaves) void k_sweep_strip2(const T* __restrict__ rho, const T* __restrict__ v,
+                                                               T* __restrict__ vout, int nx, int ny, int nz, int zs,
+                                                               T h) {
+  strip2_body<T, r2<T>()>(blockIdx.x, blockIdx.y, blockIdx.z, zs, rho, v, vout, nx, ny, nz, h);
+}
+
+template <typename T>
+inline int pick_zs2(int nx, int ny, int nz, int fixed) {
+  if (fixed > 0) return std::min(fixed, nz);
+  const int tiles = ((nx + Strip2<r2<T>()>::W - 1) / Strip2<r2<T>()>::W) *
+                    ((ny + Strip2<r2<T>()>::TY - 1) / Strip2<r2<T>()>::TY);
+  const int tz = std::max(1, (1024 + tiles - 1) / tiles);
+  return std::max(4, (nz + tz - 1) / tz);
+}
+
 // planes per workgroup: about 1024 workgroups (4096 waves) for the chip, at least 4 planes each;
 // a positive `fixed` (the tuning knob) overrides
 inline int pick_zs(int nx, int ny, int nz, int fixed) {
@@ -267,6 +416,28 @@ int sweep(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const T* rho, const T* a
   return QM_OK;
 }
 
+// two sweeps a -> b in one pass (bitwise the same as sweep(a -> t), sweep(t -> b))
+template <typename T>
+int sweep2(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const T* rho, const T* a, T* b, bool prof = true) {
+  ProfScope ps(ctx, s, QM_K_JACOBI, 2, prof);
+  constexpr int R = r2<T>();
+  const int zs = pick_zs2<T>(g->nx, g->ny, g->nz, ctx->jacobi_zs);
+  const dim3 grid((g->nx + Strip2<R>::W - 1) / Strip2<R>::W, (g->ny + Strip2<R>::TY - 1) / Strip2<R>::TY,
+                  (g->nz + zs - 1) / zs);
+  hipLaunchKernelGGL(k_sweep_strip2<T>, grid, dim3(kSW, kSWaves), 0, s, rho, a, b, g->nx, g->ny, g->nz, zs,
+                     (T)g->step);
+  QM_CHECK_LAUNCH(ctx);
+  return QM_OK;
+}
+
+// A block of n sweeps from buffer a that ends in a, with its last two sweeps
+// single (so a and the other buffer hold consecutive iterates): P two-sweep
+// passes (P even) and S = n - 2P >= 2 single sweeps.
+inline void split_block(int64_t n, int64_t& P, int64_t& S) {
+  P = n >= 2 ? 2 * ((n - 2) / 4) : 0;
+  S = n - 2 * P;
+}
+
 int host_delta(qm_ctx* ctx, hipStream_t s, int dtype, const void* a, const void* b, int64_t n, double* out) {
   double* d = nullptr;
   int rc = qm_ws(ctx, WS_SCALAR, 256, (void**)&d);
@@ -338,10 +509,16 @@ int jacobi_fp64(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho,
   int rc;
   if (max_sweeps >= block) {
     GraphBlock gb(ctx, s);
+    int64_t P, S;
+    split_block(block, P, S);
     if ((rc = gb.capture([&](hipStream_t t) -> int {
-           for (int64_t q = 0; q < block; q += 2) {
-             int r1 = sweep<double>(ctx, t, g, rho, v, tmp, false);
-             if (r1) return r1;
+           int r1;
+           for (int64_t q = 0; q < P; q += 2) {
+             if ((r1 = sweep2<double>(ctx, t, g, rho, v, tmp, false))) return r1;
+             if ((r1 = sweep2<double>(ctx, t, g, rho, tmp, v, false))) return r1;
+           }
+           for (int64_t q = 0; q < S; q += 2) {
+             if ((r1 = sweep<double>(ctx, t, g, rho, v, tmp, false))) return r1;
              if ((r1 = sweep<double>(ctx, t, g, rho, tmp, v, false))) return r1;
            }
            return QM_OK;
@@ -406,7 +583,14 @@ int jacobi_fp32(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho,
     hipLaunchKernelGGL(k_fill0, dim3(upd_blocks), dim3(256), 0, q, e, t, n);
     float* a = e;
     float* b = t;
-    for (int64_t j = 0; j < m; ++j) {
+    int64_t P, S;
+    split_block(m, P, S);
+    for (int64_t j = 0; j < P; ++j) {
+      int r1 = sweep2<float>(ctx, q, g, r, a, b, prof);
+      if (r1) return r1;
+      std::swap(a, b);
+    }
+    for (int64_t j = 0; j < S; ++j) {
       int r1 = sweep<float>(ctx, q, g, r, a, b, prof);
       if (r1) return r1;
       std::swap(a, b);
@@ -547,6 +731,7 @@ struct JItem {
   float* r32;
   int blk0, bx, by;          // pointwise kernels: 64 x 4 tiles, one z plane per block
   int blk0s, tbx, tby, zs;   // sweep: 64 x 32 tiles (four 64 x 8 strips), zs planes per block
+  int blk0t, tbx2, tby2, zs2;  // two-sweep pass: 62-column strips
 };
 
 __device__ __forceinline__ int find_item(const JItem* __restrict__ it, int n, int b) {
@@ -577,6 +762,28 @@ __device__ __forceinline__ int find_item_s(const JItem* __restrict__ it, int n, 
     else hi = mid - 1;
   }
   return lo;
+}
+
+__device__ __forceinline__ int find_item_t(const JItem* __restrict__ it, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (it[mid].blk0t <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSW* kSWaves) void kb_sweep2(const JItem* __restrict__ items, int n, int dir) {
+  const JItem& q = items[find_item_t(items, n, blockIdx.x)];
+  const int local = blockIdx.x - q.blk0t;
+  const int bx = local % q.tbx2;
+  const int rest = local / q.tbx2;
+  const int by = rest % q.tby2;
+  const int bz = rest / q.tby2;
+  strip2_body<T, r2<T>()>(bx, by, bz, q.zs2, (const T*)q.rho, (const T*)(dir ? q.b : q.a), (T*)(dir ? q.a : q.b),
+                          q.nx, q.ny, q.nz, (T)q.h);
 }
 
 template <typename T>
@@ -666,7 +873,7 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
   qm_reclaim_graphs(ctx, false);
   const bool f64 = precision == QM_PREC_FP64;
   std::vector<JItem> items(n);
-  int blocks = 0, sblocks = 0;
+  int blocks = 0, sblocks = 0, tblocks = 0;
   size_t f32_elems = 0;
   for (int i = 0; i < n; ++i) {
     if ((rc = qm_check_geom(ctx, &geoms[i]))) return rc;
@@ -684,6 +891,15 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
     q.tby = (q.ny + kTileY - 1) / kTileY;
     q.blk0s = sblocks;
     sblocks += q.tbx * q.tby * ((q.nz + q.zs - 1) / q.zs);
+    {
+      constexpr int RD = kR2D, RF = kR2F;
+      const int W = Strip2<RD>::W, TY = f64 ? Strip2<RD>::TY : Strip2<RF>::TY;
+      q.zs2 = f64 ? pick_zs2<double>(q.nx, q.ny, q.nz, ctx->jacobi_zs) : pick_zs2<float>(q.nx, q.ny, q.nz, ctx->jacobi_zs);
+      q.tbx2 = (q.nx + W - 1) / W;
+      q.tby2 = (q.ny + TY - 1) / TY;
+      q.blk0t = tblocks;
+      tblocks += q.tbx2 * q.tby2 * ((q.nz + q.zs2 - 1) / q.zs2);
+    }
     const int64_t nb = (int64_t)q.bx * q.by * q.nz;
     QM_REQUIRE(ctx, blocks + nb < 0x7fffffff, QM_ELIMIT, "jacobi_batch: batch too large");
     blocks += (int)nb;
@@ -717,19 +933,28 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
   QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, items.data(), sizeof(JItem) * items.size(), hipMemcpyHostToDevice, s));
   QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // pageable source
   const JItem* dt = (const JItem*)dtab;
-  const dim3 grid(blocks), sgrid(sblocks), thr(kBX, kBY), sthr(kSW, kSWaves);
+  const dim3 grid(blocks), sgrid(sblocks), tgrid(tblocks), thr(kBX, kBY), sthr(kSW, kSWaves);
   // one block of work on stream t; kf = sweeps in the block
   auto enqueue_block = [&](hipStream_t t, int64_t kf) -> int {
+    // kf / 2 two-sweep passes and kf % 2 single sweeps; dir = 1 while the result is in b
+    int dir = 0;
     if (f64) {
-      for (int64_t q = 0; q < kf; ++q) {
-        hipLaunchKernelGGL(kb_sweep<double>, sgrid, sthr, 0, t, dt, n, (int)(q & 1));
+      for (int64_t q = 0; q < kf / 2; ++q, dir ^= 1)
+        hipLaunchKernelGGL(kb_sweep2<double>, tgrid, sthr, 0, t, dt, n, dir);
+      if (kf & 1) {
+        hipLaunchKernelGGL(kb_sweep<double>, sgrid, sthr, 0, t, dt, n, dir);
+        dir ^= 1;
       }
-      if (kf & 1) hipLaunchKernelGGL(kb_copy_back, grid, thr, 0, t, dt, n);
+      if (dir) hipLaunchKernelGGL(kb_copy_back, grid, thr, 0, t, dt, n);
     } else {
       hipLaunchKernelGGL(kb_residual, grid, thr, 0, t, dt, n);
-      for (int64_t q = 0; q < kf; ++q)
-        hipLaunchKernelGGL(kb_sweep<float>, sgrid, sthr, 0, t, dt, n, (int)(q & 1));
-      hipLaunchKernelGGL(kb_add, grid, thr, 0, t, dt, n, (int)(kf & 1));
+      for (int64_t q = 0; q < kf / 2; ++q, dir ^= 1)
+        hipLaunchKernelGGL(kb_sweep2<float>, tgrid, sthr, 0, t, dt, n, dir);
+      if (kf & 1) {
+        hipLaunchKernelGGL(kb_sweep<float>, sgrid, sthr, 0, t, dt, n, dir);
+        dir ^= 1;
+      }
+      hipLaunchKernelGGL(kb_add, grid, thr, 0, t, dt, n, dir);
     }
     return hipGetLastError() == hipSuccess ? QM_OK : QM_EHIP;
   };

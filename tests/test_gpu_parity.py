@@ -165,6 +165,36 @@ def test_vne_classic(qm, small):
 
 
 # ---------------------------------------------------------------- Jacobi
+@pytest.mark.parametrize("precision,sweeps", [("fp64", 250), ("fp64", 7), ("fp32", 250), ("fp32", 53)])
+def test_jacobi_ragged_grid(qm, precision, sweeps):
+    """qm_jacobi on a 70 x 13 x 9 grid (no strip, tile or slab multiple) with a held nonzero outer layer:
+    two-sweep passes, single sweeps and graph replays against the oracle's plain sweeps."""
+    import ctypes
+    import torch
+    from qmtools import _native as nat
+    from oracle import jacobi as oj
+    nx, ny, nz, h = 70, 13, 9, 0.3
+    rng = numpy.random.default_rng(5)
+    rho = rng.standard_normal(nx * ny * nz)
+    bvals = rng.standard_normal(nx * ny * nz)
+    v0 = numpy.where(oj.boundary_mask(nx, ny, nz), bvals, 0.0)
+    _say("jacobi ragged", precision, sweeps)
+    dr = torch.tensor(rho, device="cuda")
+    dv = torch.tensor(v0, device="cuda")
+    dt = dv.clone()
+    done, delta = ctypes.c_int64(0), ctypes.c_double(0)
+    g = nat.geom((nx, ny, nz, h, (0.0, 0.0, 0.0)))
+    if precision == "fp64":
+        args = (nat.QM_PREC_FP64, nat.dptr(dr), nat.dptr(dv), nat.dptr(dt), sweeps, -1.0, 0)
+    else:
+        args = (nat.QM_PREC_FP32, nat.dptr(dr), nat.dptr(dv), None, sweeps, -1.0, 50)
+    nat.call("qm_jacobi", nat.stream(), ctypes.byref(g), *args, ctypes.byref(done), ctypes.byref(delta))
+    want = oj.jacobi(rho, nx, ny, nz, h, sweeps, bvals=bvals, v0=v0)
+    assert done.value == sweeps
+    assert _rel(dv.cpu().numpy(), want) <= (1e-12 if precision == "fp64" else 1e-6)
+
+
+
 @pytest.mark.parametrize("boundary", ["zero", "multipole"])
 def test_jacobi_fp64_fixed_sweeps(qm, small, boundary):
     from oracle import density as od

@@ -84,6 +84,9 @@ __device__ __forceinline__ void chunk_point(const GeomView& g, int64_t v0, int n
 
 // exp(-a r2) == 0 in double for a r2 > 745.2: a shell whose smallest exponent
 // passes that bound contributes exact zeros, so skipping it changes nothing.
+// exponent beyond which a primitive's value (e^-50 ~ 2e-22 of its coefficient) is not evaluated
+constexpr double kExpCut = 50.0;
+
 __device__ __forceinline__ void phi_f64_body(int blk, const MolView& m, const GeomView& g, int64_t v0, int nvox,
                                              int nsub, int npts, const double* __restrict__ pts, int ld,
                                              int nbf_pad, double* __restrict__ phi) {
@@ -109,7 +112,8 @@ __device__ __forceinline__ void phi_f64_body(int blk, const MolView& m, const Ge
     double radial = 0.0;
     for (int p = 0; p < sh.nprim; ++p) {
       const double2 pc = m.prims[sh.poff + p];
-      radial += pc.y * exp(-pc.x * r2);
+      const double e = pc.x * r2;
+      if (e < kExpCut) radial += pc.y * exp(-e);   // beyond: below 2e-22 of the coefficient
     }
     double v[5];
     const int nc = shell_values<double>(sh.L, dx, dy, dz, radial, v);

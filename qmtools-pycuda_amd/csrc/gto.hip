@@ -38,6 +38,7 @@ constexpr int kBoysPts = 401;            // T = 0, 0.1, ..., 40
 constexpr double kBoysStep = 0.1;
 constexpr double kBoysTmax = 40.0;
 constexpr int kThreads = 256;
+constexpr double kGtoScreen = 1e-17;      // largest dropped contribution of one distribution (hartree)
 
 __host__ __device__ constexpr int ntuv(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
 __host__ __device__ constexpr int tuv_index(int t, int u, int v) {
@@ -360,6 +361,19 @@ int build_distributions(qm_ctx* ctx, qm_mol* mol, GtoCache& c) {
               }
             }
           }
+          // Screening: |sum_tuv D_tuv R_tuv(C)| <= sum_tuv |D_tuv| (2p)^{n/2} 2^n n! for every point C
+          // (n = t+u+v; Hermite-function bound on the derivatives of F_0), so a distribution whose
+          // bound is below kGtoScreen changes no potential by more than that.
+          double bound = 0.0;
+          for (int t = 0; t <= L; ++t)
+            for (int u = 0; t + u <= L; ++u)
+              for (int v = 0; t + u + v <= L; ++v) {
+                const int n = t + u + v;
+                double g = std::pow(2.0 * p, 0.5 * n);
+                for (int k = 1; k <= n; ++k) g *= 2.0 * k;
+                bound += std::fabs(h.D[tuv_index(t, u, v)]) * g;
+              }
+          if (bound < kGtoScreen) continue;
           c.lists[L].push_back(h);
         }
       }

@@ -578,53 +578,72 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
 // are scaled per point and rounded to hi + lo while they are staged; the M
 // planes are copied as stored (K8-interleaved).  X = Ah Bh + Ah Bl + Al Bh in
 // FP32, row-dot with the FP32 basis values in FP64.
-template <typename T16>
+template <typename T16, int WR, int WC>
 __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __restrict__ phi, int ld,
                                                  const float* __restrict__ pscale, const T16* __restrict__ p_hi,
                                                  const T16* __restrict__ p_lo, int np, double* __restrict__ part,
                                                  int ldp) {
+  // WR x WC waves; wave (wr, wc) owns (128/WR) x (128/WC) outputs
   typedef typename Vec8<T16>::type V8;
-  __shared__ uint4 Ah[2][4][kTT], Al[2][4][kTT], Bh[2][4][kTT], Bl[2][4][kTT];
-  __shared__ double red[2][kTT];
+  constexpr int NTH = 64 * WR * WC;
+  constexpr int TRW = kTT / WR, TCW = kTT / WC, RT = TRW / 16, CT = TCW / 16;
+  constexpr int NA = 4 * kTT / NTH;        // (row, k group) pairs staged per thread and stage
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  __shared__ u32x4 Ah[2][4][kTT], Al[2][4][kTT], Bh[2][4][kTT], Bl[2][4][kTT];
+  __shared__ double red[WC][kTT];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int wr = wave >> 1, wc = wave & 1;
+  const int wr = wave / WC, wc = wave % WC;
   const int li = lane & 15, lk = lane >> 4;
   const int row0 = rt * kTT, c0 = cb * kTT;
   const int nst = min(np, c0 + kTT) / 32;
-  // staging: thread -> row / column ii of the tile and k group g (8 k values of a stage)
-  const int ii = t % kTT, g = t / kTT;
-  const float sc = pscale[row0 + ii];
-  const float* asrc = phi + (int64_t)(8 * g) * ld + row0 + ii;
-  const int64_t bcol = min(c0 + ii, np - 1);   // columns past the matrix are never used
-  float pa[8];
-  uint4 pbh, pbl;
-  auto load = [&](int st) {
+  // staging: pair e = t + NTH j -> row / column ii = e % 128 of the tile and k group g = e / 128
+  float sc[NA];
+  const float* asrc[NA];
+  int64_t bcol[NA];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) pa[q] = asrc[(int64_t)(st * 32 + q) * ld];
-    const int64_t o = ((int64_t)(st * 4 + g) * np + bcol) * 8;
-    pbh = *(const uint4*)(p_hi + o);
-    pbl = *(const uint4*)(p_lo + o);
-  };
-  auto store = [&](int buf) {
-    V8 h, l;
+  for (int j = 0; j < NA; ++j) {
+    const int e = t + NTH * j, ii = e % kTT, g = e / kTT;
+    sc[j] = pscale[row0 + ii];
+    asrc[j] = phi + (int64_t)(8 * g) * ld + row0 + ii;
+    bcol[j] = min(c0 + ii, np - 1);   // columns past the matrix are never used
+  }
+  float pa[NA * 8];
+  u32x4 pbh[NA], pbl[NA];
+  auto load = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const float x = pa[q] * sc;
-      const T16 hq = (T16)x;
-      h[q] = hq;
-      l[q] = (T16)(x - (float)hq);
+    for (int j = 0; j < NA; ++j) {
+      const int g = (t + NTH * j) / kTT;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) pa[8 * j + q] = asrc[j][(int64_t)(st * 32 + q) * ld];
+      const int64_t o = ((int64_t)(st * 4 + g) * np + bcol[j]) * 8;
+      pbh[j] = *(const u32x4*)(p_hi + o);
+      pbl[j] = *(const u32x4*)(p_lo + o);
     }
-    Ah[buf][g][ii] = *(const uint4*)&h;
-    Al[buf][g][ii] = *(const uint4*)&l;
-    Bh[buf][g][ii] = pbh;
-    Bl[buf][g][ii] = pbl;
   };
-  f4 acc[2][4];
+  auto store = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+    for (int j = 0; j < NA; ++j) {
+      const int e = t + NTH * j, ii = e % kTT, g = e / kTT;
+      V8 h, l;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = f4{0, 0, 0, 0};
+      for (int q = 0; q < 8; ++q) {
+        const float x = pa[8 * j + q] * sc[j];
+        const T16 hq = (T16)x;
+        h[q] = hq;
+        l[q] = (T16)(x - (float)hq);
+      }
+      Ah[buf][g][ii] = __builtin_bit_cast(u32x4, h);
+      Al[buf][g][ii] = __builtin_bit_cast(u32x4, l);
+      Bh[buf][g][ii] = pbh[j];
+      Bl[buf][g][ii] = pbl[j];
+    }
+  };
+  f4 acc[RT][CT];
+#pragma unroll
+  for (int r = 0; r < RT; ++r)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[r][c] = f4{0, 0, 0, 0};
   load(0);
   store(0);
   __syncthreads();
@@ -632,23 +651,21 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
     const bool more = st + 1 < nst;
     if (more) load(st + 1);
     const int buf = st & 1;
-    V8 ah[2], al[2], bh[4], bl[4];
+    V8 ah[RT], al[RT], bh[CT], bl[CT];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const uint4 uh = Ah[buf][lk][wr * 32 + r * 16 + li], ul = Al[buf][lk][wr * 32 + r * 16 + li];
-      ah[r] = *(const V8*)&uh;
-      al[r] = *(const V8*)&ul;
+    for (int r = 0; r < RT; ++r) {
+      ah[r] = __builtin_bit_cast(V8, Ah[buf][lk][wr * TRW + r * 16 + li]);
+      al[r] = __builtin_bit_cast(V8, Al[buf][lk][wr * TRW + r * 16 + li]);
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const uint4 uh = Bh[buf][lk][wc * 64 + c * 16 + li], ul = Bl[buf][lk][wc * 64 + c * 16 + li];
-      bh[c] = *(const V8*)&uh;
-      bl[c] = *(const V8*)&ul;
+    for (int c = 0; c < CT; ++c) {
+      bh[c] = __builtin_bit_cast(V8, Bh[buf][lk][wc * TCW + c * 16 + li]);
+      bl[c] = __builtin_bit_cast(V8, Bl[buf][lk][wc * TCW + c * 16 + li]);
     }
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RT; ++r)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < CT; ++c) {
         acc[r][c] = Vec8<T16>::mma(al[r], bh[c], acc[r][c]);
         acc[r][c] = Vec8<T16>::mma(ah[r], bl[c], acc[r][c]);
         acc[r][c] = Vec8<T16>::mma(ah[r], bh[c], acc[r][c]);
@@ -657,13 +674,13 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
     __syncthreads();
   }
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
+  for (int r = 0; r < RT; ++r) {
     double p4[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int n = c0 + wc * 64 + c * 16 + li;
+    for (int c = 0; c < CT; ++c) {
+      const int n = c0 + wc * TCW + c * 16 + li;
       if (n < np) {
-        const float4 v = *(const float4*)(phi + (int64_t)n * ld + row0 + wr * 32 + r * 16 + 4 * lk);
+        const float4 v = *(const float4*)(phi + (int64_t)n * ld + row0 + wr * TRW + r * 16 + 4 * lk);
         p4[0] += (double)(acc[r][c][0] * v.x);
         p4[1] += (double)(acc[r][c][1] * v.y);
         p4[2] += (double)(acc[r][c][2] * v.z);
@@ -673,22 +690,27 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const double v = sum16(p4[q]);
-      if (li == 0) red[wc][wr * 32 + r * 16 + 4 * lk + q] = v;
+      if (li == 0) red[wc][wr * TRW + r * 16 + 4 * lk + q] = v;
     }
   }
   __syncthreads();
-  if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
+  for (int e = t; e < kTT; e += NTH) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < WC; ++w) v += red[w][e];
+    part[(int64_t)cb * ldp + row0 + e] = v;
+  }
 }
 
-template <typename T16>
-__global__ __launch_bounds__(kTWG) void k_contract_split_tiles(const float* __restrict__ phi, int ld,
-                                                               const float* __restrict__ pscale,
-                                                               const T16* __restrict__ p_hi,
-                                                               const T16* __restrict__ p_lo, int np, int nrt, int ncb,
-                                                               double* __restrict__ part) {
+template <typename T16, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const float* __restrict__ phi, int ld,
+                                                                      const float* __restrict__ pscale,
+                                                                      const T16* __restrict__ p_hi,
+                                                                      const T16* __restrict__ p_lo, int np, int nrt,
+                                                                      int ncb, double* __restrict__ part) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  tiled_split_body<T16>(rt, cb, phi, ld, pscale, p_hi, p_lo, np, part, ld);
+  tiled_split_body<T16, WR, WC>(rt, cb, phi, ld, pscale, p_hi, p_lo, np, part, ld);
 }
 
 __global__ __launch_bounds__(kTWG) void k_contract_f64_tiles(const double* __restrict__ phi, int ld,
@@ -1071,7 +1093,7 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
   }
   const T16* Phi = (const T16*)(std::is_same<T16, _Float16>::value ? mol->P16 : mol->Pb16);
   const T16* Plo = Phi + (int64_t)np * np;
-  const bool tiled = ctx->split_tm == 0;
+  const bool tiled = ctx->split_tm == 0 || ctx->split_tm == 2;
   const int ncb = (np + kTT - 1) / kTT;
   double* part = nullptr;
   if (tiled && (rc = qm_ws(ctx, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
@@ -1091,8 +1113,12 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
     if (tiled) {
       const int nrt = ld / kTT;
-      hipLaunchKernelGGL(k_contract_split_tiles<T16>, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const float*)phi,
-                         ld, (const float*)pscale, Phi, Plo, np, nrt, ncb, part);
+      if (ctx->split_tm == 2)
+        hipLaunchKernelGGL((k_contract_split_tiles<T16, 2, 2>), dim3(tile_grid(nrt, ncb)), dim3(256), 0, s,
+                           (const float*)phi, ld, (const float*)pscale, Phi, Plo, np, nrt, ncb, part);
+      else
+        hipLaunchKernelGGL((k_contract_split_tiles<T16, 4, 2>), dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s,
+                           (const float*)phi, ld, (const float*)pscale, Phi, Plo, np, nrt, ncb, part);
       QM_CHECK_LAUNCH(ctx);
       if (nsub3 == 1) {
         hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncb,
@@ -1193,8 +1219,8 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
   int rc = qm_enter(ctx);
   if (rc) return rc;
   if (knob == QM_TUNE_SPLIT_VARIANT) {
-    QM_REQUIRE(ctx, value == 0 || value == 64 || value == 128, QM_EINVAL,
-               "tune: split variant must be 0 (tiles), 64 or 128 (rows per workgroup)");
+    QM_REQUIRE(ctx, value == 0 || value == 2 || value == 64 || value == 128, QM_EINVAL,
+               "tune: split variant must be 0 or 2 (tiles, 8 or 4 waves), 64 or 128 (rows per workgroup)");
     ctx->split_tm = value;
     return QM_OK;
   }
@@ -1243,7 +1269,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   void* out = nullptr;
   if ((rc = qm_ws(ctx, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
-  std::vector<int> cands = split ? std::vector<int>{0, 64, 128} : std::vector<int>{0, 1, 2};
+  std::vector<int> cands = split ? std::vector<int>{0, 2, 64, 128} : std::vector<int>{0, 1, 2};
   int& knob = split ? ctx->split_tm : ctx->f64_variant;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));
@@ -1409,13 +1435,13 @@ __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __re
   tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 
-template <typename T16>
-__global__ __launch_bounds__(kTWG) void kb_contract_split_tiles(const BChunk* __restrict__ c, int n) {
+template <typename T16, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void kb_contract_split_tiles(const BChunk* __restrict__ c, int n) {
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
-  tiled_split_body<T16>(rt, cb, (const float*)q.phi, q.ld, q.pscale, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad,
-                        q.part, q.ld);
+  tiled_split_body<T16, WR, WC>(rt, cb, (const float*)q.phi, q.ld, q.pscale, (const T16*)q.ph, (const T16*)q.pl,
+                                q.nbf_pad, q.part, q.ld);
 }
 
 // same sums as k_tiles_finish; one thread per point, blocks numbered like the basis-value launch
@@ -1456,7 +1482,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   const bool f64 = precision == QM_PREC_FP64;
   const size_t elt = f64 ? sizeof(double) : sizeof(float);
   const int tm = f64 ? 64 : ctx->split_tm;
-  const bool tiled = f64 ? ctx->f64_variant == 2 : ctx->split_tm == 0;
+  const bool tiled = f64 ? ctx->f64_variant == 2 : (ctx->split_tm == 0 || ctx->split_tm == 2);
   // chunks per molecule, packed into rounds of at most kPhiBudget bytes of basis values
   std::vector<BChunk> table;
   std::vector<Round> rounds;
@@ -1581,10 +1607,14 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       if (tiled) {
         if (f64)
           hipLaunchKernelGGL(kb_contract_f64_tiles, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+        else if (precision == QM_PREC_F16X3 && ctx->split_tm == 2)
+          hipLaunchKernelGGL((kb_contract_split_tiles<_Float16, 2, 2>), dim3(blocks), dim3(256), 0, s, gc, gn);
         else if (precision == QM_PREC_F16X3)
-          hipLaunchKernelGGL(kb_contract_split_tiles<_Float16>, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+          hipLaunchKernelGGL((kb_contract_split_tiles<_Float16, 4, 2>), dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+        else if (ctx->split_tm == 2)
+          hipLaunchKernelGGL((kb_contract_split_tiles<__bf16, 2, 2>), dim3(blocks), dim3(256), 0, s, gc, gn);
         else
-          hipLaunchKernelGGL(kb_contract_split_tiles<__bf16>, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+          hipLaunchKernelGGL((kb_contract_split_tiles<__bf16, 4, 2>), dim3(blocks), dim3(kTWG), 0, s, gc, gn);
         QM_CHECK_LAUNCH(ctx);
         // finish over the whole round: chunk blocks numbered as in the basis-value launch
         if (out_dtype == QM_F64)

@@ -30,6 +30,7 @@ constexpr double kRBohr = 0.529177211;       // kernel_fft.cu:15
 constexpr double kEps0 = 0.0055263494;       // kernel_fft.cu:16, e/(V Angstrom)
 constexpr double kAng2Bor = 1.8897259886;
 constexpr int kThreads = 256;
+constexpr int kFftThreads = 512;   // threads of an axis-pass workgroup (B n <= 4096: 8 points per thread)
 constexpr int kMaxStages = 24;
 
 template <typename R>
@@ -72,6 +73,29 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
     x[1] = Cx<R>{a1.x + b1w.x, a1.y + b1w.y};
     x[2] = Cx<R>{a0.x - b0.x, a0.y - b0.y};
     x[3] = Cx<R>{a1.x - b1w.x, a1.y - b1w.y};
+  } else if (RR == 8) {
+    // X_2r = DFT4(x_m + x_m+4)_r, X_2r+1 = DFT4((x_m - x_m+4) W8^m)_r; W8^m = tw[m w1], W4 = W8^2
+    Cx<R> a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = Cx<R>{x[q].x + x[q + 4].x, x[q].y + x[q + 4].y};
+      b[q] = Cx<R>{x[q].x - x[q + 4].x, x[q].y - x[q + 4].y};
+    }
+    b[1] = cmul(b[1], tw[w1]);
+    b[2] = cmul(b[2], tw[2 * w1]);
+    b[3] = cmul(b[3], tw[3 * w1]);
+    const Cx<R> w4 = tw[2 * w1];
+    auto dft4 = [&](const Cx<R>* c, Cx<R>& o0, Cx<R>& o1, Cx<R>& o2, Cx<R>& o3) {
+      const Cx<R> d0{c[0].x + c[2].x, c[0].y + c[2].y}, d1{c[0].x - c[2].x, c[0].y - c[2].y};
+      const Cx<R> d2{c[1].x + c[3].x, c[1].y + c[3].y};
+      const Cx<R> d3 = cmul(Cx<R>{c[1].x - c[3].x, c[1].y - c[3].y}, w4);
+      o0 = Cx<R>{d0.x + d2.x, d0.y + d2.y};
+      o1 = Cx<R>{d1.x + d3.x, d1.y + d3.y};
+      o2 = Cx<R>{d0.x - d2.x, d0.y - d2.y};
+      o3 = Cx<R>{d1.x - d3.x, d1.y - d3.y};
+    };
+    dft4(a, x[0], x[2], x[4], x[6]);
+    dft4(b, x[1], x[3], x[5], x[7]);
   } else {
     Cx<R> y[RR];
 #pragma unroll
@@ -94,14 +118,14 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
 // are read into registers, then (after a barrier) the outputs are written.
 template <typename R, int RR>
 __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restrict__ tw, int n, int B, int p) {
-  constexpr int MAXB = (16 + RR - 1) / RR;   // B * n <= 4096: at most ceil(16/RR) butterflies per thread
+  constexpr int MAXB = (4096 / kFftThreads + RR - 1) / RR;   // B * n <= 4096: at most ceil(8/RR) butterflies per thread
   const int m = n / RR;
   const int tstep = n / (p * RR);
   const int total = B * m;
   Cx<R> x[MAXB][RR];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
-    const int bf = threadIdx.x + b * kThreads;
+    const int bf = threadIdx.x + b * kFftThreads;
     if (bf < total) {
       const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
       const Cx<R>* src = buf + l * n;
@@ -117,7 +141,7 @@ __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restric
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
-    const int bf = threadIdx.x + b * kThreads;
+    const int bf = threadIdx.x + b * kFftThreads;
     if (bf < total) {
       const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
       Cx<R>* dst = buf + l * n + (i - k) * RR + k;
@@ -134,7 +158,7 @@ __device__ __forceinline__ void stage_generic(const Cx<R>* in, Cx<R>* out, const
                                               int B, int RR, int p) {
   const int m = n / RR;
   const int tstep = n / (p * RR);
-  for (int bf = threadIdx.x; bf < B * m; bf += kThreads) {
+  for (int bf = threadIdx.x; bf < B * m; bf += kFftThreads) {
     const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
     const Cx<R>* src = in + l * n;
     Cx<R>* dst = out + l * n + (i - k) * RR + k;
@@ -155,8 +179,10 @@ __device__ __forceinline__ void stage_generic(const Cx<R>* in, Cx<R>* out, const
 
 // One FFT pass along one axis.  INV: inverse (conjugated) transform, unnormalised.
 // inplace: all radices <= 8 (LDS = table + B lines); otherwise two planes of B lines.
-template <typename R, bool INV>
-__global__ __launch_bounds__(kThreads) void k_fft_axis(Cx<R>* __restrict__ data, const Cx<R>* __restrict__ twg,
+// POW2: the length is a power of two (radices 8, 4, 2 only), which keeps the
+// register footprint of the butterflies, and so the kernel's, small.
+template <typename R, bool INV, bool POW2>
+__global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ data, const Cx<R>* __restrict__ twg,
                                                        AxisPlan plan, AxisLines ln, int inplace) {
   extern __shared__ unsigned char smem[];
   const int n = plan.n;
@@ -168,29 +194,59 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(Cx<R>* __restrict__ data,
   const int64_t wg = blockIdx.x;
   const int64_t l0 = wg / lines_per_row;
   const int l1b = (int)(wg % lines_per_row) * B;
-  for (int q = threadIdx.x; q < n; q += kThreads) {
+  for (int q = threadIdx.x; q < n; q += kFftThreads) {
     Cx<R> w = twg[q];
     if (INV) w.y = -w.y;
     tw[q] = w;
   }
   const int64_t base0 = l0 * ln.s0 + (int64_t)l1b * ln.s1;
   const int nb = min(B, ln.nl1 - l1b);
-  for (int e = threadIdx.x; e < B * n; e += kThreads) {
-    int l, a;
-    if (ln.stride == 1) {
-      l = e / n;
-      a = e - l * n;
-    } else {
-      l = e % B;
-      a = e / B;
+  // all of a thread's loads are issued before any is used (B n <= 4096: 8 per thread)
+  constexpr int kPer = 4096 / kFftThreads;
+  Cx<R> ld[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int e = threadIdx.x + q * kFftThreads;
+    ld[q] = Cx<R>{0, 0};
+    if (e < B * n) {
+      int l, a;
+      if (ln.stride == 1) {
+        l = e / n;
+        a = e - l * n;
+      } else {
+        l = e % B;
+        a = e / B;
+      }
+      if (l < nb) ld[q] = data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride];
     }
-    Cx<R> v{0, 0};
-    if (l < nb) v = data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride];
-    buf0[l * n + a] = v;
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int e = threadIdx.x + q * kFftThreads;
+    if (e < B * n) {
+      int l, a;
+      if (ln.stride == 1) {
+        l = e / n;
+        a = e - l * n;
+      } else {
+        l = e % B;
+        a = e / B;
+      }
+      buf0[l * n + a] = ld[q];
+    }
   }
   __syncthreads();
   Cx<R>* res = buf0;
-  if (inplace) {
+  if (POW2) {
+    for (int s = 0; s < plan.nstage; ++s) {
+      const int p = plan.span[s];
+      switch (plan.radix[s]) {
+        case 2: stage_inplace<R, 2>(buf0, tw, n, B, p); break;
+        case 4: stage_inplace<R, 4>(buf0, tw, n, B, p); break;
+        default: stage_inplace<R, 8>(buf0, tw, n, B, p); break;
+      }
+    }
+  } else if (inplace) {
     for (int s = 0; s < plan.nstage; ++s) {
       const int p = plan.span[s];
       switch (plan.radix[s]) {
@@ -213,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(Cx<R>* __restrict__ data,
     }
     res = in;
   }
-  for (int e = threadIdx.x; e < B * n; e += kThreads) {
+  for (int e = threadIdx.x; e < B * n; e += kFftThreads) {
     int l, a;
     if (ln.stride == 1) {
       l = e / n;
@@ -431,15 +487,22 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
   const size_t shmem = sizeof(Cx<R>) * (size_t)n * (1 + (size_t)planes * B);
   QM_REQUIRE(ctx, nwg <= 0x7fffffff, QM_ELIMIT, "fft: too many lines");
   if (nwg == 0) return QM_OK;
+  const bool pow2 = (n & (n - 1)) == 0;
   static bool attr_set = false;
   if (!attr_set) {
-    QM_CHECK_HIP(ctx, hipFuncSetAttribute((const void*)k_fft_axis<R, INV>,
+    QM_CHECK_HIP(ctx, hipFuncSetAttribute((const void*)k_fft_axis<R, INV, false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+    QM_CHECK_HIP(ctx, hipFuncSetAttribute((const void*)k_fft_axis<R, INV, true>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
     attr_set = true;
   }
   ProfScope ps(ctx, s, QM_K_FFT_AXIS);
-  hipLaunchKernelGGL((k_fft_axis<R, INV>), dim3((unsigned)nwg), dim3(kThreads), shmem, s, data, tw, plan, ln,
-                     inplace ? 1 : 0);
+  if (pow2)
+    hipLaunchKernelGGL((k_fft_axis<R, INV, true>), dim3((unsigned)nwg), dim3(kFftThreads), shmem, s, data, tw, plan, ln,
+                       1);
+  else
+    hipLaunchKernelGGL((k_fft_axis<R, INV, false>), dim3((unsigned)nwg), dim3(kFftThreads), shmem, s, data, tw, plan,
+                       ln, inplace ? 1 : 0);
   QM_CHECK_LAUNCH(ctx);
   return QM_OK;
 }

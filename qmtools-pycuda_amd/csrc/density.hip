@@ -713,6 +713,105 @@ __global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const flo
   tiled_split_body<T16, WR, WC>(rt, cb, phi, ld, pscale, p_hi, p_lo, np, part, ld);
 }
 
+// Variant 3: tiles of 128 points x 64 columns with waves stacked 8 x 1 (wave w
+// owns rows 16w .. 16w+15 and all 64 columns), so every wave has the same
+// columns and, in the stages that cross the block's diagonal, all waves skip
+// the same MFMAs: tile column c (columns cs .. cs+15) takes the k-substep
+// starting at k only if k <= cs + 15 (M is zero below the diagonal).  The
+// narrower blocks also follow the triangle more closely; row-dots complete
+// inside a wave.  Column blocks of 64: ncb = nbf_pad / 64.
+constexpr int kRN = 64;
+
+__device__ __forceinline__ void tiled_f64_rows_body(int rt, int cb, const double* __restrict__ phi, int ld,
+                                                    const double* __restrict__ M, int np, double* __restrict__ part,
+                                                    int ldp) {
+  constexpr int CT = kRN / 16;
+  __shared__ double As[2][kTKS][kTT + kTPad];
+  __shared__ double Bs[2][kTKS][kRN + kTPad];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  const int row0 = rt * kTT, c0 = cb * kRN;
+  const int nst = min(np, c0 + kRN) / kTKS;
+  const int nfull = c0 / kTKS;                   // stages below the block: every tile takes them
+  constexpr int QA = kTKS * kTT / kTWG;          // 4 basis values per thread and stage
+  constexpr int KA = kTWG / kTT;
+  const int kka = t / kTT, ia = t % kTT;
+  const int kkb = t / kRN, ib = t % kRN;         // 2 x 16 x 64: threads < 1024 -> one M value each (t < 1024 always)
+  const unsigned aoff = (unsigned)(kka * ld + ia);
+  const double* abase = phi + row0;
+  const int bcol = min(c0 + ib, np - 1);         // columns past the matrix are never used
+  double pa[QA], pb[2];
+  auto load = [&](int st) {
+#pragma unroll
+    for (int q = 0; q < QA; ++q) pa[q] = (abase + (int64_t)(st * kTKS + KA * q) * ld)[aoff];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) pb[q] = M[(int64_t)(st * kTKS + kkb + 8 * q) * np + bcol];
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < QA; ++q) As[buf][kka + KA * q][ia] = pa[q];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) Bs[buf][kkb + 8 * q][ib] = pb[q];
+  };
+  d4 acc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) acc[c] = d4{0, 0, 0, 0};
+  // one stage on the tiles c >= C (straight-line code: no branch around an MFMA)
+  auto stage = [&](auto cfirst, int buf) {
+    constexpr int C = decltype(cfirst)::value;
+#pragma unroll
+    for (int ks = 0; ks < kTKS; ks += 4) {
+      const double a = As[buf][ks + lk][wave * 16 + li];
+      double b[CT];
+#pragma unroll
+      for (int c = C; c < CT; ++c) b[c] = Bs[buf][ks + lk][c * 16 + li];
+#pragma unroll
+      for (int c = C; c < CT; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[c], acc[c], 0, 0, 0);
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const bool more = st + 1 < nst;
+    if (more) load(st + 1);
+    const int buf = st & 1;
+    // stage j = st - nfull of the diagonal: tiles c < j lie below it for every k of the stage
+    switch (st < nfull ? 0 : st - nfull) {
+      case 0: stage(std::integral_constant<int, 0>{}, buf); break;
+      case 1: stage(std::integral_constant<int, 1>{}, buf); break;
+      case 2: stage(std::integral_constant<int, 2>{}, buf); break;
+      default: stage(std::integral_constant<int, 3>{}, buf); break;
+    }
+    if (more) store((st + 1) & 1);
+    __syncthreads();
+  }
+  double p4[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int n = c0 + c * 16 + li;
+    if (n < np) {
+      const double* ph = phi + (int64_t)n * ld + row0 + wave * 16 + lk;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) p4[q] += acc[c][q] * ph[4 * q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double v = sum16(p4[q]);
+    if (li == 0) part[(int64_t)cb * ldp + row0 + wave * 16 + 4 * q + lk] = v;
+  }
+}
+
+__global__ __launch_bounds__(kTWG) void k_contract_f64_rows(const double* __restrict__ phi, int ld,
+                                                            const double* __restrict__ M, int np, int nrt, int ncb,
+                                                            double* __restrict__ part) {
+  int rt, cb;
+  if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
+  tiled_f64_rows_body(rt, cb, phi, ld, M, np, part, ld);
+}
+
 __global__ __launch_bounds__(kTWG) void k_contract_f64_tiles(const double* __restrict__ phi, int ld,
                                                              const double* __restrict__ M, int np, int nrt, int ncb,
                                                              double* __restrict__ part) {
@@ -1021,8 +1120,8 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
     rc = qm_ws(ctx, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
     if (rc) return rc;
   }
-  const bool tiled = ctx->f64_variant == 2;
-  const int ncb = (np + kTT - 1) / kTT;
+  const bool tiled = ctx->f64_variant == 2 || ctx->f64_variant == 3;
+  const int ncb = ctx->f64_variant == 3 ? np / kRN : (np + kTT - 1) / kTT;
   double* part = nullptr;
   if (tiled && (rc = qm_ws(ctx, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
   const GeomView gv = geom_view(g);
@@ -1041,8 +1140,12 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
     ProfScope pc(ctx, s, QM_K_DENSITY_CONTRACT);
     if (tiled) {
       const int nrt = ld / kTT;
-      hipLaunchKernelGGL(k_contract_f64_tiles, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const double*)phi, ld,
-                         (const double*)mol->M64, np, nrt, ncb, part);
+      if (ctx->f64_variant == 3)
+        hipLaunchKernelGGL(k_contract_f64_rows, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const double*)phi, ld,
+                           (const double*)mol->M64, np, nrt, ncb, part);
+      else
+        hipLaunchKernelGGL(k_contract_f64_tiles, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const double*)phi, ld,
+                           (const double*)mol->M64, np, nrt, ncb, part);
       QM_CHECK_LAUNCH(ctx);
       if (nsub3 == 1) {
         hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncb,
@@ -1225,7 +1328,7 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
     return QM_OK;
   }
   if (knob == QM_TUNE_F64_VARIANT) {
-    QM_REQUIRE(ctx, value >= 0 && value <= 2, QM_EINVAL, "tune: f64 variant must be 0, 1 or 2");
+    QM_REQUIRE(ctx, value >= 0 && value <= 3, QM_EINVAL, "tune: f64 variant must be 0, 1, 2 or 3");
     ctx->f64_variant = value;
     return QM_OK;
   }
@@ -1269,7 +1372,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   void* out = nullptr;
   if ((rc = qm_ws(ctx, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
-  std::vector<int> cands = split ? std::vector<int>{0, 2, 64, 128} : std::vector<int>{0, 1, 2};
+  std::vector<int> cands = split ? std::vector<int>{0, 2, 64, 128} : std::vector<int>{0, 1, 2, 3};
   int& knob = split ? ctx->split_tm : ctx->f64_variant;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));
@@ -1428,6 +1531,13 @@ void launch_batch_split(hipStream_t s, int nt, int tm, const BChunk* c, int n, i
 #undef QM_BSPLIT
 }
 
+__global__ __launch_bounds__(kTWG) void kb_contract_f64_rows(const BChunk* __restrict__ c, int n) {
+  const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
+  int rt, cb;
+  if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
+  tiled_f64_rows_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
+}
+
 __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __restrict__ c, int n) {
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
@@ -1482,7 +1592,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   const bool f64 = precision == QM_PREC_FP64;
   const size_t elt = f64 ? sizeof(double) : sizeof(float);
   const int tm = f64 ? 64 : ctx->split_tm;
-  const bool tiled = f64 ? ctx->f64_variant == 2 : (ctx->split_tm == 0 || ctx->split_tm == 2);
+  const bool tiled = f64 ? (ctx->f64_variant == 2 || ctx->f64_variant == 3) : (ctx->split_tm == 0 || ctx->split_tm == 2);
   // chunks per molecule, packed into rounds of at most kPhiBudget bytes of basis values
   std::vector<BChunk> table;
   std::vector<Round> rounds;
@@ -1536,7 +1646,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       c.ld = (int)round_up(c.nvox, kTileM);
       c.nbf_pad = np;
       c.nt = tiled ? 0 : pick_nt(np);     // tiled chunks form one launch per round
-      c.ncb = (np + kTT - 1) / kTT;
+      c.ncb = (f64 && ctx->f64_variant == 3) ? np / kRN : (np + kTT - 1) / kTT;
       const size_t bytes = (size_t)np * c.ld * elt;
       if (cur_bytes + bytes > kPhiBudget) flush();
       c.phi = (void*)(uintptr_t)cur_bytes;            // offsets for now, rebased below
@@ -1605,7 +1715,9 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       const int blocks = r.group_blocks[gi];
       ProfScope pc(ctx, s, f64 ? QM_K_DENSITY_CONTRACT : QM_K_DENSITY_SPLIT);
       if (tiled) {
-        if (f64)
+        if (f64 && ctx->f64_variant == 3)
+          hipLaunchKernelGGL(kb_contract_f64_rows, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+        else if (f64)
           hipLaunchKernelGGL(kb_contract_f64_tiles, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
         else if (precision == QM_PREC_F16X3 && ctx->split_tm == 2)
           hipLaunchKernelGGL((kb_contract_split_tiles<_Float16, 2, 2>), dim3(blocks), dim3(256), 0, s, gc, gn);

@@ -202,6 +202,10 @@ int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr,
       delete m;
       return qm_fail(ctx, QM_EINVAL, "shell %d: bad table entry", s);
     }
+    double amin = alpha[d.poff];
+    for (int p = 1; p < d.nprim; ++p) amin = std::min(amin, alpha[d.poff + p]);
+    // rounded up, so a skipped shell has alpha r^2 > kPhiExpCut for every primitive
+    d.rcut2 = amin > 0.0 ? std::nextafter((float)(kPhiExpCut / amin), INFINITY) : INFINITY;
     bf += 2 * d.L + 1;
     m->h_shells[s] = d;
   }

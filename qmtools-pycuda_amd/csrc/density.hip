@@ -85,7 +85,7 @@ __device__ __forceinline__ void chunk_point(const GeomView& g, int64_t v0, int n
 // exp(-a r2) == 0 in double for a r2 > 745.2: a shell whose smallest exponent
 // passes that bound contributes exact zeros, so skipping it changes nothing.
 // exponent beyond which a primitive's value (e^-50 ~ 2e-22 of its coefficient) is not evaluated
-constexpr double kExpCut = 50.0;
+constexpr double kExpCut = kPhiExpCut;
 
 __device__ __forceinline__ void phi_f64_body(int blk, const MolView& m, const GeomView& g, int64_t v0, int nvox,
                                              int nsub, int npts, const double* __restrict__ pts, int ld,
@@ -110,10 +110,12 @@ __device__ __forceinline__ void phi_f64_body(int blk, const MolView& m, const Ge
     const double dx = x - a.x, dy = y - a.y, dz = z - a.z;
     const double r2 = dx * dx + dy * dy + dz * dz;
     double radial = 0.0;
-    for (int p = 0; p < sh.nprim; ++p) {
-      const double2 pc = m.prims[sh.poff + p];
-      const double e = pc.x * r2;
-      if (e < kExpCut) radial += pc.y * exp(-e);   // beyond: below 2e-22 of the coefficient
+    if (r2 <= (double)sh.rcut2) {
+      for (int p = 0; p < sh.nprim; ++p) {
+        const double2 pc = m.prims[sh.poff + p];
+        const double e = pc.x * r2;
+        if (e < kExpCut) radial += pc.y * exp(-e);   // beyond: below 2e-22 of the coefficient
+      }
     }
     double v[5];
     const int nc = shell_values<double>(sh.L, dx, dy, dz, radial, v);
@@ -162,9 +164,11 @@ __device__ __forceinline__ void phi_f32_body(int blk, const MolView& m, const fl
     const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
     const float r2 = dx * dx + dy * dy + dz * dz;
     float radial = 0.f;
-    for (int p = 0; p < sh.nprim; ++p) {
-      const float2 pc = prim_l2e[sh.poff + p];
-      radial += pc.y * __builtin_amdgcn_exp2f(-pc.x * r2);
+    if (r2 <= sh.rcut2) {
+      for (int p = 0; p < sh.nprim; ++p) {
+        const float2 pc = prim_l2e[sh.poff + p];
+        radial += pc.y * __builtin_amdgcn_exp2f(-pc.x * r2);
+      }
     }
     float v[5];
     const int nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
@@ -184,6 +188,86 @@ __global__ __launch_bounds__(kPhiThreads) void k_phi_f32(MolView m, const float2
                                                          int ld, int nbf_pad, float* __restrict__ phi,
                                                          float* __restrict__ pscale) {
   phi_f32_body(blockIdx.x, m, prim_l2e, g, v0, nvox, nsub, npts, ld, nbf_pad, phi, pscale);
+}
+
+// Basis values already scaled and split for the 16-bit contraction: the hi and
+// lo planes of (phi S) in the K8-interleaved operand layout, element (f, t) at
+// ((f / 8) ld + t) 8 + f % 8, in one pass.  S is one power of two per molecule
+// (split_phi_scale) that keeps every |phi S| <= 2^14, so hi never overflows FP16
+// and values down to 2^-14 / S keep the full 22 bits of hi + lo; smaller ones
+// (points where every basis function is that small) lose low bits to FP16
+// subnormals, far below the density's relative tolerance.  Each group of 8
+// functions is packed in registers and stored as one 16-byte vector per plane.
+// Same arithmetic for the values as phi_f32_body.
+template <typename T16>
+__device__ __forceinline__ uint32_t bits16(T16 v) { return (uint32_t)__builtin_bit_cast(uint16_t, v); }
+
+template <typename T16>
+__device__ __forceinline__ void phi_split_body(int blk, const MolView& m, const float2* __restrict__ prim_l2e,
+                                               const GeomView& g, int64_t v0, int nvox, int nsub, int npts, int ld,
+                                               int nbf_pad, float S, T16* __restrict__ hi, T16* __restrict__ lo) {
+  const int t = blk * kPhiThreads + threadIdx.x;
+  if (t >= ld) return;
+  double x = 0, y = 0, z = 0;
+  if (t < npts) chunk_point(g, v0, nvox, nsub, t, x, y, z);
+  const float sc = t < npts ? S : 0.f;   // padding points store zeros through the same (uniform) loop
+  uint32_t h[4] = {0, 0, 0, 0}, l[4] = {0, 0, 0, 0};
+  int f = 0;   // same sequence in every lane
+  auto push = [&](float val) __attribute__((always_inline)) {
+    const float xs = val * sc;
+    const T16 hq = (T16)xs;
+    const uint32_t hb = bits16(hq), lb = bits16((T16)(xs - (float)hq));
+    const int pos = f & 7;
+    switch (pos >> 1) {   // constant register indices: no indexed register access
+      case 0:
+        h[0] = (pos & 1) ? (h[0] | (hb << 16)) : hb;
+        l[0] = (pos & 1) ? (l[0] | (lb << 16)) : lb;
+        break;
+      case 1:
+        h[1] = (pos & 1) ? (h[1] | (hb << 16)) : hb;
+        l[1] = (pos & 1) ? (l[1] | (lb << 16)) : lb;
+        break;
+      case 2:
+        h[2] = (pos & 1) ? (h[2] | (hb << 16)) : hb;
+        l[2] = (pos & 1) ? (l[2] | (lb << 16)) : lb;
+        break;
+      default:
+        h[3] = (pos & 1) ? (h[3] | (hb << 16)) : hb;
+        l[3] = (pos & 1) ? (l[3] | (lb << 16)) : lb;
+        break;
+    }
+    if (pos == 7) {
+      const int64_t o = ((int64_t)(f >> 3) * ld + t) * 8;
+      *(uint4*)(hi + o) = uint4{h[0], h[1], h[2], h[3]};
+      *(uint4*)(lo + o) = uint4{l[0], l[1], l[2], l[3]};
+    }
+    ++f;
+  };
+  for (int s = 0; s < m.nshell; ++s) {
+    const DevShell sh = m.shells[s];
+    const double4 a = m.atoms[sh.atom];
+    const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float radial = 0.f;
+    if (r2 <= sh.rcut2) {
+      for (int p = 0; p < sh.nprim; ++p) {
+        const float2 pc = prim_l2e[sh.poff + p];
+        radial += pc.y * __builtin_amdgcn_exp2f(-pc.x * r2);
+      }
+    }
+    float v[5];
+    const int nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
+    for (int c = 0; c < nc; ++c) push(v[c]);
+  }
+  while (f < nbf_pad) push(0.f);
+}
+
+template <typename T16>
+__global__ __launch_bounds__(kPhiThreads) void k_phi_split(MolView m, const float2* __restrict__ prim_l2e, GeomView g,
+                                                           int64_t v0, int nvox, int nsub, int npts, int ld,
+                                                           int nbf_pad, float S, T16* __restrict__ hi,
+                                                           T16* __restrict__ lo) {
+  phi_split_body<T16>(blockIdx.x, m, prim_l2e, g, v0, nvox, nsub, npts, ld, nbf_pad, S, hi, lo);
 }
 
 // sum over the 16 lanes that share lane/16 (the n index of the MFMA D layout)
@@ -578,18 +662,33 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
 // are scaled per point and rounded to hi + lo while they are staged; the M
 // planes are copied as stored (K8-interleaved).  X = Ah Bh + Ah Bl + Al Bh in
 // FP32, row-dot with the FP32 basis values in FP64.
-template <typename T16, int WR, int WC>
-__device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __restrict__ phi, int ld,
+// AM (how the basis-value operand A reaches the MFMAs):
+//  0: FP32 values, scaled and split while they are staged into LDS;
+//  1: values already scaled and split (k_phi_split: hi plane, then lo plane,
+//     np ld values each, K8-interleaved), staged into LDS as plain 16-byte copies;
+//  2: as 1, but every wave loads its A fragments straight from global memory into
+//     registers (one 16-byte load per lane, fragment and plane), one stage ahead;
+//     only M goes through LDS, which halves the LDS traffic per MFMA.
+// With AM >= 1 the row-dot uses hi + lo, and the finishing pass divides by the
+// molecule's S^2 (folded into its scale) instead of the per-point pscale.
+template <typename T16, int WR, int WC, int AM = 0>
+__device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __restrict__ phiv, int ld,
                                                  const float* __restrict__ pscale, const T16* __restrict__ p_hi,
                                                  const T16* __restrict__ p_lo, int np, double* __restrict__ part,
                                                  int ldp) {
+  constexpr bool PRE = AM >= 1;
+  constexpr bool AD = AM == 2;
+  const float* __restrict__ phi = (const float*)phiv;
+  const T16* __restrict__ a_hi = (const T16*)phiv;
+  const T16* __restrict__ a_lo = a_hi + (int64_t)np * ld;
   // WR x WC waves; wave (wr, wc) owns (128/WR) x (128/WC) outputs
   typedef typename Vec8<T16>::type V8;
   constexpr int NTH = 64 * WR * WC;
   constexpr int TRW = kTT / WR, TCW = kTT / WC, RT = TRW / 16, CT = TCW / 16;
   constexpr int NA = 4 * kTT / NTH;        // (row, k group) pairs staged per thread and stage
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-  __shared__ u32x4 Ah[2][4][kTT], Al[2][4][kTT], Bh[2][4][kTT], Bl[2][4][kTT];
+  __shared__ u32x4 Ah[AD ? 1 : 2][AD ? 1 : 4][AD ? 1 : kTT], Al[AD ? 1 : 2][AD ? 1 : 4][AD ? 1 : kTT];
+  __shared__ u32x4 Bh[2][4][kTT], Bl[2][4][kTT];
   __shared__ double red[WC][kTT];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -604,18 +703,35 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
 #pragma unroll
   for (int j = 0; j < NA; ++j) {
     const int e = t + NTH * j, ii = e % kTT, g = e / kTT;
-    sc[j] = pscale[row0 + ii];
+    if (!PRE) sc[j] = pscale[row0 + ii];
     asrc[j] = phi + (int64_t)(8 * g) * ld + row0 + ii;
     bcol[j] = min(c0 + ii, np - 1);   // columns past the matrix are never used
   }
-  float pa[NA * 8];
-  u32x4 pbh[NA], pbl[NA];
+  float pa[PRE ? 1 : NA * 8];
+  u32x4 pah[NA], pal[NA], pbh[NA], pbl[NA];
+  // AD: fragment r of this wave = rows row0 + wr TRW + 16 r + li, k group st 4 + lk
+  u32x4 fah[RT], fal[RT], nah[RT], nal[RT];
+  auto load_frag = [&](int st, u32x4* h, u32x4* l) __attribute__((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+      const int64_t oa = ((int64_t)(st * 4 + lk) * ld + row0 + wr * TRW + r * 16 + li) * 8;
+      h[r] = *(const u32x4*)(a_hi + oa);
+      l[r] = *(const u32x4*)(a_lo + oa);
+    }
+  };
   auto load = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
-      const int g = (t + NTH * j) / kTT;
+      const int e = t + NTH * j, ii = e % kTT, g = e / kTT;
+      if (AD) {
+      } else if (PRE) {
+        const int64_t oa = ((int64_t)(st * 4 + g) * ld + row0 + ii) * 8;
+        pah[j] = *(const u32x4*)(a_hi + oa);
+        pal[j] = *(const u32x4*)(a_lo + oa);
+      } else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) pa[8 * j + q] = asrc[j][(int64_t)(st * 32 + q) * ld];
+        for (int q = 0; q < 8; ++q) pa[8 * j + q] = asrc[j][(int64_t)(st * 32 + q) * ld];
+      }
       const int64_t o = ((int64_t)(st * 4 + g) * np + bcol[j]) * 8;
       pbh[j] = *(const u32x4*)(p_hi + o);
       pbl[j] = *(const u32x4*)(p_lo + o);
@@ -625,16 +741,22 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       const int e = t + NTH * j, ii = e % kTT, g = e / kTT;
-      V8 h, l;
+      if (AD) {
+      } else if (PRE) {
+        Ah[buf][g][ii] = pah[j];
+        Al[buf][g][ii] = pal[j];
+      } else {
+        V8 h, l;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const float x = pa[8 * j + q] * sc[j];
-        const T16 hq = (T16)x;
-        h[q] = hq;
-        l[q] = (T16)(x - (float)hq);
+        for (int q = 0; q < 8; ++q) {
+          const float x = pa[8 * j + q] * sc[j];
+          const T16 hq = (T16)x;
+          h[q] = hq;
+          l[q] = (T16)(x - (float)hq);
+        }
+        Ah[buf][g][ii] = __builtin_bit_cast(u32x4, h);
+        Al[buf][g][ii] = __builtin_bit_cast(u32x4, l);
       }
-      Ah[buf][g][ii] = __builtin_bit_cast(u32x4, h);
-      Al[buf][g][ii] = __builtin_bit_cast(u32x4, l);
       Bh[buf][g][ii] = pbh[j];
       Bl[buf][g][ii] = pbl[j];
     }
@@ -644,18 +766,27 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
   for (int r = 0; r < RT; ++r)
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[r][c] = f4{0, 0, 0, 0};
+  if (AD) load_frag(0, fah, fal);
   load(0);
   store(0);
   __syncthreads();
   for (int st = 0; st < nst; ++st) {
     const bool more = st + 1 < nst;
-    if (more) load(st + 1);
+    if (more) {
+      load(st + 1);
+      if (AD) load_frag(st + 1, nah, nal);
+    }
     const int buf = st & 1;
     V8 ah[RT], al[RT], bh[CT], bl[CT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
-      ah[r] = __builtin_bit_cast(V8, Ah[buf][lk][wr * TRW + r * 16 + li]);
-      al[r] = __builtin_bit_cast(V8, Al[buf][lk][wr * TRW + r * 16 + li]);
+      if (AD) {
+        ah[r] = __builtin_bit_cast(V8, fah[r]);
+        al[r] = __builtin_bit_cast(V8, fal[r]);
+      } else {
+        ah[r] = __builtin_bit_cast(V8, Ah[buf][lk][wr * TRW + r * 16 + li]);
+        al[r] = __builtin_bit_cast(V8, Al[buf][lk][wr * TRW + r * 16 + li]);
+      }
     }
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
@@ -670,7 +801,16 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
         acc[r][c] = Vec8<T16>::mma(ah[r], bl[c], acc[r][c]);
         acc[r][c] = Vec8<T16>::mma(ah[r], bh[c], acc[r][c]);
       }
-    if (more) store((st + 1) & 1);
+    if (more) {
+      store((st + 1) & 1);
+      if (AD) {
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+          fah[r] = nah[r];
+          fal[r] = nal[r];
+        }
+      }
+    }
     __syncthreads();
   }
 #pragma unroll
@@ -680,11 +820,21 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
     for (int c = 0; c < CT; ++c) {
       const int n = c0 + wc * TCW + c * 16 + li;
       if (n < np) {
-        const float4 v = *(const float4*)(phi + (int64_t)n * ld + row0 + wr * TRW + r * 16 + 4 * lk);
-        p4[0] += (double)(acc[r][c][0] * v.x);
-        p4[1] += (double)(acc[r][c][1] * v.y);
-        p4[2] += (double)(acc[r][c][2] * v.z);
-        p4[3] += (double)(acc[r][c][3] * v.w);
+        if (PRE) {
+          // element (n, t) of a plane sits at ((n / 8) ld + t) 8 + n % 8
+          const int64_t o = ((int64_t)(n >> 3) * ld + row0 + wr * TRW + r * 16 + 4 * lk) * 8 + (n & 7);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float v = (float)a_hi[o + 8 * q] + (float)a_lo[o + 8 * q];
+            p4[q] += (double)(acc[r][c][q] * v);
+          }
+        } else {
+          const float4 v = *(const float4*)(phi + (int64_t)n * ld + row0 + wr * TRW + r * 16 + 4 * lk);
+          p4[0] += (double)(acc[r][c][0] * v.x);
+          p4[1] += (double)(acc[r][c][1] * v.y);
+          p4[2] += (double)(acc[r][c][2] * v.z);
+          p4[3] += (double)(acc[r][c][3] * v.w);
+        }
       }
     }
 #pragma unroll
@@ -702,15 +852,138 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const float* __
   }
 }
 
-template <typename T16, int WR, int WC>
-__global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const float* __restrict__ phi, int ld,
+template <typename T16, int WR, int WC, int AM = 0>
+__global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const void* __restrict__ phi, int ld,
                                                                       const float* __restrict__ pscale,
                                                                       const T16* __restrict__ p_hi,
                                                                       const T16* __restrict__ p_lo, int np, int nrt,
                                                                       int ncb, double* __restrict__ part) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  tiled_split_body<T16, WR, WC>(rt, cb, phi, ld, pscale, p_hi, p_lo, np, part, ld);
+  tiled_split_body<T16, WR, WC, AM>(rt, cb, phi, ld, pscale, p_hi, p_lo, np, part, ld);
+}
+
+// Tiled split contraction on pre-split basis values with both operands
+// fetched by LDS DMA (global_load_lds_dwordx4: each lane's 16 bytes land at
+// M0 + 16 lane, so a wave fills 1 KB of LDS without touching registers) into a
+// ring of NB stages of 32 k; stage st + NB - 1 is requested while stage st is
+// multiplied, so NB - 1 stages of loads are in flight behind the MFMAs.  The
+// ring slot written at stage st was last read at stage st - 1, which every wave
+// has finished once it passes stage st's barrier.  The loads are issued from
+// inline assembly (M0 holds the LDS address) so that the compiler does not wait
+// for all of them before each LDS read; the explicit s_waitcnt vmcnt before the
+// barrier waits for exactly the oldest stage.  No other code in this kernel uses
+// M0.  Tiles, partial sums and the epilogue are those of tiled_split_body (AM 1).
+__device__ __forceinline__ void lds_dma16(const void* gsrc, const void* lds_wave_base) {
+  const unsigned l = __builtin_amdgcn_readfirstlane(
+      (unsigned)(uintptr_t)(__attribute__((address_space(3))) const void*)lds_wave_base);
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(l), "v"(gsrc) : "memory");
+}
+
+// wait until at most 4 k of this thread's DMA loads are outstanding (4 per stage)
+__device__ __forceinline__ void dma_wait_stages(int k) {
+  if (k >= 2) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+  else if (k == 1) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+  else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <typename T16, int NB>
+__device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __restrict__ a_hi, int ld,
+                                               const T16* __restrict__ p_hi, const T16* __restrict__ p_lo, int np,
+                                               double* __restrict__ part, int ldp) {
+  static_assert(NB >= 2 && NB <= 4, "ring of 2..4 stages");
+  typedef typename Vec8<T16>::type V8;
+  constexpr int WR = 4, WC = 2, NTH = 512, TRW = kTT / WR, TCW = kTT / WC, RT = TRW / 16, CT = TCW / 16;
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  // ring slot: 4 operand planes (A hi, A lo, B hi, B lo) of [4 k groups][128] 16-byte words
+  __shared__ u32x4 ring[NB][4][4][kTT];
+  __shared__ double red[WC][kTT];
+  const T16* __restrict__ a_lo = a_hi + (int64_t)np * ld;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wr = wave / WC, wc = wave % WC;
+  const int li = lane & 15, lk = lane >> 4;
+  const int row0 = rt * kTT, c0 = cb * kTT;
+  const int nst = min(np, c0 + kTT) / 32;
+  // thread t fetches word t of each plane: k group g = t / 128, row / column ii = t % 128
+  const int g = t / kTT, ii = t % kTT;
+  const int bcol = min(c0 + ii, np - 1);   // columns past the matrix are never used
+  auto issue = [&](int st) __attribute__((always_inline)) {
+    const int slot = st % NB;
+    const int64_t oa = ((int64_t)(st * 4 + g) * ld + row0 + ii) * 8;
+    const int64_t ob = ((int64_t)(st * 4 + g) * np + bcol) * 8;
+    lds_dma16(a_hi + oa, &ring[slot][0][0][0] + 64 * wave);
+    lds_dma16(a_lo + oa, &ring[slot][1][0][0] + 64 * wave);
+    lds_dma16(p_hi + ob, &ring[slot][2][0][0] + 64 * wave);
+    lds_dma16(p_lo + ob, &ring[slot][3][0][0] + 64 * wave);
+  };
+  f4 acc[RT][CT];
+#pragma unroll
+  for (int r = 0; r < RT; ++r)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[r][c] = f4{0, 0, 0, 0};
+#pragma unroll
+  for (int st = 0; st < NB - 1; ++st)
+    if (st < nst) issue(st);
+  for (int st = 0; st < nst; ++st) {
+    dma_wait_stages(min(NB - 2, nst - 1 - st));   // stages st+1 .. st+NB-2 may stay in flight
+    __syncthreads();
+    if (st + NB - 1 < nst) issue(st + NB - 1);
+    const int slot = st % NB;
+    V8 ah[RT], al[RT], bh[CT], bl[CT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+      ah[r] = __builtin_bit_cast(V8, ring[slot][0][lk][wr * TRW + r * 16 + li]);
+      al[r] = __builtin_bit_cast(V8, ring[slot][1][lk][wr * TRW + r * 16 + li]);
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      bh[c] = __builtin_bit_cast(V8, ring[slot][2][lk][wc * TCW + c * 16 + li]);
+      bl[c] = __builtin_bit_cast(V8, ring[slot][3][lk][wc * TCW + c * 16 + li]);
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        acc[r][c] = Vec8<T16>::mma(al[r], bh[c], acc[r][c]);
+        acc[r][c] = Vec8<T16>::mma(ah[r], bl[c], acc[r][c]);
+        acc[r][c] = Vec8<T16>::mma(ah[r], bh[c], acc[r][c]);
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    double p4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int n = c0 + wc * TCW + c * 16 + li;
+      if (n < np) {
+        // element (n, t) of a plane sits at ((n / 8) ld + t) 8 + n % 8
+        const int64_t o = ((int64_t)(n >> 3) * ld + row0 + wr * TRW + r * 16 + 4 * lk) * 8 + (n & 7);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float v = (float)a_hi[o + 8 * q] + (float)a_lo[o + 8 * q];
+          p4[q] += (double)(acc[r][c][q] * v);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double v = sum16(p4[q]);
+      if (li == 0) red[wc][wr * TRW + r * 16 + 4 * lk + q] = v;
+    }
+  }
+  __syncthreads();
+  if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
+}
+
+template <typename T16, int NB>
+__global__ __launch_bounds__(512) void k_contract_split_dma(const void* __restrict__ phi, int ld,
+                                                            const T16* __restrict__ p_hi,
+                                                            const T16* __restrict__ p_lo, int np, int nrt, int ncb,
+                                                            double* __restrict__ part) {
+  int rt, cb;
+  if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
+  tiled_dma_body<T16, NB>(rt, cb, (const T16*)phi, ld, p_hi, p_lo, np, part, ld);
 }
 
 // Variant 3: tiles of 128 points x 64 columns with waves stacked 8 x 1 (wave w
@@ -1172,6 +1445,57 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
   return QM_OK;
 }
 
+// Power of two S with |phi S| <= 2^14 for every basis function at every point:
+// a contracted shell is bounded by sum_p |c_p| g_L(alpha_p), with the largest
+// angular factor times max_r r^L exp(-a r^2): g_0 = C00, g_1 = C1 / sqrt(2 e a),
+// g_2 = max(2 C20, C22, C2A / 2) / (e a).
+float split_phi_scale(const qm_mol* m) {
+  const double g2 = std::max(std::max(2.0 * C20, C22), 0.5 * C2A);
+  double bound = 0.0;
+  for (const DevShell& sh : m->h_shells) {
+    double b = 0.0;
+    for (int p = 0; p < sh.nprim; ++p) {
+      const double a = m->h_alpha[sh.poff + p], c = std::fabs(m->h_coeff[sh.poff + p]);
+      b += c * (sh.L == 0 ? C00 : sh.L == 1 ? C1 / std::sqrt(2.0 * M_E * a) : g2 / (M_E * a));
+    }
+    bound = std::max(bound, b);
+  }
+  if (!(bound > 0.0) || !std::isfinite(bound)) return 1.0f;
+  int e = 0;
+  std::frexp(bound, &e);   // bound < 2^e
+  return std::ldexp(1.0f, std::max(-100, std::min(100, 14 - e)));
+}
+
+// Split variants of the contraction: 0 / 2 tiles of 8 (4 x 2) / 4 (2 x 2) waves on FP32
+// basis values; 1 / 3 the same on pre-split values staged through LDS; 4 / 5 on
+// pre-split values loaded straight into registers; 6 / 7 / 8 on pre-split values
+// fetched by LDS DMA into a ring of 2 / 3 / 4 stages; 64 / 128 all-columns kernels.
+inline bool split_tiled(int v) { return v >= 0 && v <= 8; }
+inline bool split_pre(int v) { return v >= 1 && v <= 8 && v != 2; }
+
+template <typename T16>
+void launch_split_tiles(hipStream_t s, int v, int blocks, const void* phi, int ld, const float* pscale,
+                        const T16* Ph, const T16* Pl, int np, int nrt, int ncb, double* part) {
+#define QM_TILES(WR, WC, AM)                                                                              \
+  hipLaunchKernelGGL((k_contract_split_tiles<T16, WR, WC, AM>), dim3(blocks), dim3(64 * WR * WC), 0, s, phi, \
+                     ld, pscale, Ph, Pl, np, nrt, ncb, part)
+#define QM_DMA(NB) \
+  hipLaunchKernelGGL((k_contract_split_dma<T16, NB>), dim3(blocks), dim3(512), 0, s, phi, ld, Ph, Pl, np, nrt, ncb, part)
+  switch (v) {
+    case 1: QM_TILES(4, 2, 1); break;
+    case 2: QM_TILES(2, 2, 0); break;
+    case 3: QM_TILES(2, 2, 1); break;
+    case 4: QM_TILES(4, 2, 2); break;
+    case 5: QM_TILES(2, 2, 2); break;
+    case 6: QM_DMA(2); break;
+    case 7: QM_DMA(3); break;
+    case 8: QM_DMA(4); break;
+    default: QM_TILES(4, 2, 0); break;
+  }
+#undef QM_TILES
+#undef QM_DMA
+}
+
 template <typename T16, typename TO>
 int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int nsub, TO* out) {
   const int np = mol->nbf_pad;
@@ -1196,7 +1520,9 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
   }
   const T16* Phi = (const T16*)(std::is_same<T16, _Float16>::value ? mol->P16 : mol->Pb16);
   const T16* Plo = Phi + (int64_t)np * np;
-  const bool tiled = ctx->split_tm == 0 || ctx->split_tm == 2;
+  const bool pre = split_pre(ctx->split_tm);
+  const bool tiled = split_tiled(ctx->split_tm);
+  const float S = split_phi_scale(mol);
   const int ncb = (np + kTT - 1) / kTT;
   double* part = nullptr;
   if (tiled && (rc = qm_ws(ctx, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
@@ -1209,26 +1535,30 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     const int ld = (int)round_up(npts, kTileM);
     {
       ProfScope ps(ctx, s, QM_K_PHI_SPLIT);
-      hipLaunchKernelGGL(k_phi_f32, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s, mv,
-                         (const float2*)mol->prims_l2e, gv, v0, nv, nsub, npts, ld, np, (float*)phi, pscale);
+      // both layouts take 4 bytes per (function, point): FP32, or 16-bit hi + 16-bit lo
+      if (pre)
+        hipLaunchKernelGGL(k_phi_split<T16>, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s, mv,
+                           (const float2*)mol->prims_l2e, gv, v0, nv, nsub, npts, ld, np, S, (T16*)phi,
+                           (T16*)phi + (int64_t)np * ld);
+      else
+        hipLaunchKernelGGL(k_phi_f32, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s, mv,
+                           (const float2*)mol->prims_l2e, gv, v0, nv, nsub, npts, ld, np, (float*)phi, pscale);
     }
     QM_CHECK_LAUNCH(ctx);
     ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
     if (tiled) {
       const int nrt = ld / kTT;
-      if (ctx->split_tm == 2)
-        hipLaunchKernelGGL((k_contract_split_tiles<T16, 2, 2>), dim3(tile_grid(nrt, ncb)), dim3(256), 0, s,
-                           (const float*)phi, ld, (const float*)pscale, Phi, Plo, np, nrt, ncb, part);
-      else
-        hipLaunchKernelGGL((k_contract_split_tiles<T16, 4, 2>), dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s,
-                           (const float*)phi, ld, (const float*)pscale, Phi, Plo, np, nrt, ncb, part);
+      launch_split_tiles<T16>(s, ctx->split_tm, tile_grid(nrt, ncb), phi, ld, pscale, Phi, Plo, np, nrt, ncb, part);
       QM_CHECK_LAUNCH(ctx);
+      // pre-split: both factors of the row-dot carry S, divided out here with the volume element
+      const float* ps = pre ? nullptr : (const float*)pscale;
+      const double sc = pre ? h3 / ((double)S * S) : h3;
       if (nsub3 == 1) {
         hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncb,
-                           npts, h3, (const float*)pscale, out, v0);
+                           npts, sc, ps, out, v0);
       } else {
         hipLaunchKernelGGL(k_tiles_finish<double>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld,
-                           ncb, npts, h3 / nsub3, (const float*)pscale, rho, (int64_t)0);
+                           ncb, npts, sc / nsub3, ps, rho, (int64_t)0);
         QM_CHECK_LAUNCH(ctx);
         hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
                            nsub3, out, v0);
@@ -1322,8 +1652,8 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
   int rc = qm_enter(ctx);
   if (rc) return rc;
   if (knob == QM_TUNE_SPLIT_VARIANT) {
-    QM_REQUIRE(ctx, value == 0 || value == 2 || value == 64 || value == 128, QM_EINVAL,
-               "tune: split variant must be 0 or 2 (tiles, 8 or 4 waves), 64 or 128 (rows per workgroup)");
+    QM_REQUIRE(ctx, split_tiled(value) || value == 64 || value == 128, QM_EINVAL,
+               "tune: split variant must be 0..5 (tiles) or 64 / 128 (rows per workgroup)");
     ctx->split_tm = value;
     return QM_OK;
   }
@@ -1372,7 +1702,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   void* out = nullptr;
   if ((rc = qm_ws(ctx, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
-  std::vector<int> cands = split ? std::vector<int>{0, 2, 64, 128} : std::vector<int>{0, 1, 2, 3};
+  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 4, 5, 6, 7, 8, 64, 128} : std::vector<int>{0, 1, 2, 3};
   int& knob = split ? ctx->split_tm : ctx->f64_variant;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));
@@ -1439,6 +1769,7 @@ struct BChunk {
   void* out;
   double scale;
   double* part;         // tiled kernels: per-column-block partial sums [ncb][ld]
+  float phis;           // pre-split basis values: the molecule's scale S
   int ncb;
   int blk0_phi;         // first block in the round's basis-value (and tiled finish) launch
   int blk0;             // first block in its contraction launch
@@ -1467,6 +1798,15 @@ __global__ __launch_bounds__(kPhiThreads) void kb_phi_f32(const BChunk* __restri
   const BChunk& q = c[i];
   phi_f32_body(blockIdx.x - q.blk0_phi, q.mol, q.l2e, q.g, q.v0, q.nvox, 1, q.nvox, q.ld, q.nbf_pad, (float*)q.phi,
                q.pscale);
+}
+
+template <typename T16>
+__global__ __launch_bounds__(kPhiThreads) void kb_phi_split(const BChunk* __restrict__ c, int n) {
+  const int i = find_chunk(c, n, blockIdx.x, true);
+  const BChunk& q = c[i];
+  T16* hi = (T16*)q.phi;
+  phi_split_body<T16>(blockIdx.x - q.blk0_phi, q.mol, q.l2e, q.g, q.v0, q.nvox, 1, q.nvox, q.ld, q.nbf_pad, q.phis,
+                      hi, hi + (int64_t)q.nbf_pad * q.ld);
 }
 
 template <int NT, typename TO>
@@ -1545,13 +1885,42 @@ __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __re
   tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 
-template <typename T16, int WR, int WC>
+template <typename T16, int WR, int WC, int AM = 0>
 __global__ __launch_bounds__(64 * WR * WC) void kb_contract_split_tiles(const BChunk* __restrict__ c, int n) {
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
-  tiled_split_body<T16, WR, WC>(rt, cb, (const float*)q.phi, q.ld, q.pscale, (const T16*)q.ph, (const T16*)q.pl,
-                                q.nbf_pad, q.part, q.ld);
+  tiled_split_body<T16, WR, WC, AM>(rt, cb, q.phi, q.ld, q.pscale, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad,
+                                     q.part, q.ld);
+}
+
+template <typename T16, int NB>
+__global__ __launch_bounds__(512) void kb_contract_split_dma(const BChunk* __restrict__ c, int n) {
+  const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
+  int rt, cb;
+  if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
+  tiled_dma_body<T16, NB>(rt, cb, (const T16*)q.phi, q.ld, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad, q.part,
+                          q.ld);
+}
+
+template <typename T16>
+void launch_batch_split_tiles(hipStream_t s, int v, int blocks, const BChunk* c, int n) {
+#define QM_BTILES(WR, WC, AM) \
+  hipLaunchKernelGGL((kb_contract_split_tiles<T16, WR, WC, AM>), dim3(blocks), dim3(64 * WR * WC), 0, s, c, n)
+#define QM_BDMA(NB) hipLaunchKernelGGL((kb_contract_split_dma<T16, NB>), dim3(blocks), dim3(512), 0, s, c, n)
+  switch (v) {
+    case 1: QM_BTILES(4, 2, 1); break;
+    case 2: QM_BTILES(2, 2, 0); break;
+    case 3: QM_BTILES(2, 2, 1); break;
+    case 4: QM_BTILES(4, 2, 2); break;
+    case 5: QM_BTILES(2, 2, 2); break;
+    case 6: QM_BDMA(2); break;
+    case 7: QM_BDMA(3); break;
+    case 8: QM_BDMA(4); break;
+    default: QM_BTILES(4, 2, 0); break;
+  }
+#undef QM_BTILES
+#undef QM_BDMA
 }
 
 // same sums as k_tiles_finish; one thread per point, blocks numbered like the basis-value launch
@@ -1592,7 +1961,9 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   const bool f64 = precision == QM_PREC_FP64;
   const size_t elt = f64 ? sizeof(double) : sizeof(float);
   const int tm = f64 ? 64 : ctx->split_tm;
-  const bool tiled = f64 ? (ctx->f64_variant == 2 || ctx->f64_variant == 3) : (ctx->split_tm == 0 || ctx->split_tm == 2);
+  const bool tiled = f64 ? (ctx->f64_variant == 2 || ctx->f64_variant == 3) : split_tiled(ctx->split_tm);
+  // basis values written scaled and split (same bytes as FP32)
+  const bool pre = !f64 && split_pre(ctx->split_tm);
   // chunks per molecule, packed into rounds of at most kPhiBudget bytes of basis values
   std::vector<BChunk> table;
   std::vector<Round> rounds;
@@ -1665,7 +2036,8 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
         const uint16_t* base = (const uint16_t*)(half ? m->P16 : m->Pb16);
         c.ph = base;
         c.pl = base + (size_t)np * np;
-        c.scale = h3 / m->pfac;
+        c.phis = split_phi_scale(m);
+        c.scale = pre ? h3 / m->pfac / ((double)c.phis * c.phis) : h3 / m->pfac;
       }
       c.out = outs[i];
       cur.push_back(c);
@@ -1704,6 +2076,12 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       ProfScope pp(ctx, s, f64 ? QM_K_PHI : QM_K_PHI_SPLIT);
       if (f64)
         hipLaunchKernelGGL(kb_phi_f64, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first, (int)r.count);
+      else if (pre && precision == QM_PREC_F16X3)
+        hipLaunchKernelGGL(kb_phi_split<_Float16>, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first,
+                           (int)r.count);
+      else if (pre)
+        hipLaunchKernelGGL(kb_phi_split<__bf16>, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first,
+                           (int)r.count);
       else
         hipLaunchKernelGGL(kb_phi_f32, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first, (int)r.count);
     }
@@ -1719,22 +2097,19 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
           hipLaunchKernelGGL(kb_contract_f64_rows, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
         else if (f64)
           hipLaunchKernelGGL(kb_contract_f64_tiles, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
-        else if (precision == QM_PREC_F16X3 && ctx->split_tm == 2)
-          hipLaunchKernelGGL((kb_contract_split_tiles<_Float16, 2, 2>), dim3(blocks), dim3(256), 0, s, gc, gn);
         else if (precision == QM_PREC_F16X3)
-          hipLaunchKernelGGL((kb_contract_split_tiles<_Float16, 4, 2>), dim3(blocks), dim3(kTWG), 0, s, gc, gn);
-        else if (ctx->split_tm == 2)
-          hipLaunchKernelGGL((kb_contract_split_tiles<__bf16, 2, 2>), dim3(blocks), dim3(256), 0, s, gc, gn);
+          launch_batch_split_tiles<_Float16>(s, ctx->split_tm, blocks, gc, gn);
         else
-          hipLaunchKernelGGL((kb_contract_split_tiles<__bf16, 4, 2>), dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+          launch_batch_split_tiles<__bf16>(s, ctx->split_tm, blocks, gc, gn);
         QM_CHECK_LAUNCH(ctx);
         // finish over the whole round: chunk blocks numbered as in the basis-value launch
+        const int per_point = !f64 && !pre;   // per-point scales to divide out
         if (out_dtype == QM_F64)
           hipLaunchKernelGGL(kb_tiles_finish<double>, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first,
-                             (int)r.count, f64 ? 0 : 1);
+                             (int)r.count, per_point);
         else
           hipLaunchKernelGGL(kb_tiles_finish<float>, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first,
-                             (int)r.count, f64 ? 0 : 1);
+                             (int)r.count, per_point);
       } else if (f64) {
         if (out_dtype == QM_F64) launch_batch_f64<double>(s, nt, gc, gn, blocks, ctx->f64_variant == 1);
         else launch_batch_f64<float>(s, nt, gc, gn, blocks, ctx->f64_variant == 1);

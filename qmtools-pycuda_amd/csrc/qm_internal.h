@@ -14,13 +14,18 @@
 #include "qmgrid.h"
 
 // ------------------------------------------------------------------ device-side molecule view
+// Basis values skip a primitive where alpha r^2 > kPhiExpCut (below 2e-22 of its
+// coefficient) and a whole shell where r^2 > rcut2 = kPhiExpCut / min alpha.
+constexpr double kPhiExpCut = 50.0;
+
 struct DevShell {
-  int atom;    // index into atoms
-  int L;       // 0..2
-  int nprim;   // primitives
-  int poff;    // first primitive
-  int bf;      // first basis function
-  int pad_[3];
+  int atom;     // index into atoms
+  int L;        // 0..2
+  int nprim;    // primitives
+  int poff;     // first primitive
+  int bf;       // first basis function
+  float rcut2;  // squared distance (bohr^2) beyond which every primitive is cut
+  int pad_[2];
 };
 
 struct MolView {
@@ -95,7 +100,7 @@ struct qm_ctx {
   bool prof = false;
   hipStream_t graph_stream = nullptr;   // non-blocking stream for graph capture (default priority)
   std::vector<LiveGraph> graphs;        // graphs whose replays may still run
-  int split_tm = 0;                     // split contraction: 0 tiled, else rows per workgroup (64 or 128)
+  int split_tm = 6;                     // split contraction variant (0..8 tiled, 64 / 128 all-columns)
   int f64_variant = 2;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled
   int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)
   std::vector<ProfRec> recs;

@@ -196,9 +196,12 @@ __global__ __launch_bounds__(kPhiThreads) void k_phi_f32(MolView m, const float2
 // (split_phi_scale) that keeps every |phi S| <= 2^14, so hi never overflows FP16
 // and values down to 2^-14 / S keep the full 22 bits of hi + lo; smaller ones
 // (points where every basis function is that small) lose low bits to FP16
-// subnormals, far below the density's relative tolerance.  Each group of 8
-// functions is packed in registers and stored as one 16-byte vector per plane.
-// Same arithmetic for the values as phi_f32_body.
+// subnormals, far below the density's relative tolerance.  The (hi, lo) pair of
+// function f goes to LDS word [f % 8][thread] (consecutive lanes, consecutive
+// words: no bank conflicts, no branches on f); after every 8th function the lane
+// reads its 8 words back and stores 16 bytes per plane.  Each lane only reads
+// what it wrote, so no barrier is needed.  Same arithmetic for the values as
+// phi_f32_body.
 template <typename T16>
 __device__ __forceinline__ uint32_t bits16(T16 v) { return (uint32_t)__builtin_bit_cast(uint16_t, v); }
 
@@ -206,40 +209,36 @@ template <typename T16>
 __device__ __forceinline__ void phi_split_body(int blk, const MolView& m, const float2* __restrict__ prim_l2e,
                                                const GeomView& g, int64_t v0, int nvox, int nsub, int npts, int ld,
                                                int nbf_pad, float S, T16* __restrict__ hi, T16* __restrict__ lo) {
+  __shared__ uint32_t stage[8][kPhiThreads];
   const int t = blk * kPhiThreads + threadIdx.x;
   if (t >= ld) return;
   double x = 0, y = 0, z = 0;
   if (t < npts) chunk_point(g, v0, nvox, nsub, t, x, y, z);
   const float sc = t < npts ? S : 0.f;   // padding points store zeros through the same (uniform) loop
-  uint32_t h[4] = {0, 0, 0, 0}, l[4] = {0, 0, 0, 0};
+  uint32_t* __restrict__ mine = &stage[0][threadIdx.x];
   int f = 0;   // same sequence in every lane
   auto push = [&](float val) __attribute__((always_inline)) {
     const float xs = val * sc;
     const T16 hq = (T16)xs;
-    const uint32_t hb = bits16(hq), lb = bits16((T16)(xs - (float)hq));
-    const int pos = f & 7;
-    switch (pos >> 1) {   // constant register indices: no indexed register access
-      case 0:
-        h[0] = (pos & 1) ? (h[0] | (hb << 16)) : hb;
-        l[0] = (pos & 1) ? (l[0] | (lb << 16)) : lb;
-        break;
-      case 1:
-        h[1] = (pos & 1) ? (h[1] | (hb << 16)) : hb;
-        l[1] = (pos & 1) ? (l[1] | (lb << 16)) : lb;
-        break;
-      case 2:
-        h[2] = (pos & 1) ? (h[2] | (hb << 16)) : hb;
-        l[2] = (pos & 1) ? (l[2] | (lb << 16)) : lb;
-        break;
-      default:
-        h[3] = (pos & 1) ? (h[3] | (hb << 16)) : hb;
-        l[3] = (pos & 1) ? (l[3] | (lb << 16)) : lb;
-        break;
-    }
-    if (pos == 7) {
+    const T16 lq = (T16)(xs - (float)hq);
+    mine[(f & 7) * kPhiThreads] = bits16(hq) | (bits16(lq) << 16);
+    if ((f & 7) == 7) {
+      uint32_t w[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) w[q] = mine[q * kPhiThreads];
+      // hi halves of words (2j, 2j+1) -> H[j], lo halves -> L[j]
+      uint4 H, L;
+      H.x = __builtin_amdgcn_perm(w[1], w[0], 0x05040100u);
+      H.y = __builtin_amdgcn_perm(w[3], w[2], 0x05040100u);
+      H.z = __builtin_amdgcn_perm(w[5], w[4], 0x05040100u);
+      H.w = __builtin_amdgcn_perm(w[7], w[6], 0x05040100u);
+      L.x = __builtin_amdgcn_perm(w[1], w[0], 0x07060302u);
+      L.y = __builtin_amdgcn_perm(w[3], w[2], 0x07060302u);
+      L.z = __builtin_amdgcn_perm(w[5], w[4], 0x07060302u);
+      L.w = __builtin_amdgcn_perm(w[7], w[6], 0x07060302u);
       const int64_t o = ((int64_t)(f >> 3) * ld + t) * 8;
-      *(uint4*)(hi + o) = uint4{h[0], h[1], h[2], h[3]};
-      *(uint4*)(lo + o) = uint4{l[0], l[1], l[2], l[3]};
+      *(uint4*)(hi + o) = H;
+      *(uint4*)(lo + o) = L;
     }
     ++f;
   };
@@ -658,26 +657,51 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
 }
 
 // Tiled 16-bit split contraction: the tiling, triangle, workgroup order and
-// partial sums of the FP64 tiled kernel, with stages of 32 k.  The basis values
-// are scaled per point and rounded to hi + lo while they are staged; the M
-// planes are copied as stored (K8-interleaved).  X = Ah Bh + Ah Bl + Al Bh in
-// FP32, row-dot with the FP32 basis values in FP64.
-// AM (how the basis-value operand A reaches the MFMAs):
-//  0: FP32 values, scaled and split while they are staged into LDS;
-//  1: values already scaled and split (k_phi_split: hi plane, then lo plane,
-//     np ld values each, K8-interleaved), staged into LDS as plain 16-byte copies;
-//  2: as 1, but every wave loads its A fragments straight from global memory into
-//     registers (one 16-byte load per lane, fragment and plane), one stage ahead;
-//     only M goes through LDS, which halves the LDS traffic per MFMA.
-// With AM >= 1 the row-dot uses hi + lo, and the finishing pass divides by the
-// molecule's S^2 (folded into its scale) instead of the per-point pscale.
-template <typename T16, int WR, int WC, int AM = 0>
+// partial sums of the FP64 tiled kernel, with stages of 32 k.  The M planes are
+// copied as stored (K8-interleaved); X = Ah Bh + Ah Bl + Al Bh in FP32.
+// PRE false: FP32 basis values, scaled per point and rounded to hi + lo while
+//   they are staged into LDS; row-dot with the FP32 values in FP64.
+// PRE true: values already scaled by the molecule's S and split (k_phi_split:
+//   hi plane, then lo plane, np ld values each, K8-interleaved), staged as plain
+//   16-byte copies; the MFMAs take M as their first operand, so each 16 x 16
+//   tile of X comes out transposed (lane (li, lk), register q = X[t = li][n =
+//   4 lk + q]) and the row-dot reads its 4 consecutive functions of one point
+//   as a single 8-byte word per plane (presplit_rowdot).  The finishing pass
+//   divides by S^2 (folded into its scale) instead of the per-point pscale.
+template <typename T16, int RT, int CT, int TRW, int TCW, int WC>
+__device__ __forceinline__ void presplit_rowdot(const f4 (&acc)[RT][CT], const T16* __restrict__ a_hi,
+                                                const T16* __restrict__ a_lo, int ld, int np, int row0, int c0,
+                                                int wr, int wc, int li, int lk, double (*red)[kTT]) {
+  typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+  typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+  typedef typename std::conditional<std::is_same<T16, _Float16>::value, h4, b4>::type V4;
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    const int tl = wr * TRW + r * 16 + li;
+    double p = 0.0;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int n0 = c0 + wc * TCW + c * 16 + 4 * lk;   // a multiple of 4: n0 .. n0+3 share one 8-group
+      if (n0 < np) {
+        // element (n, t) of a plane sits at ((n / 8) ld + t) 8 + n % 8
+        const int64_t o = ((int64_t)(n0 >> 3) * ld + row0 + tl) * 8 + (n0 & 7);
+        const V4 h = __builtin_bit_cast(V4, *(const uint2*)(a_hi + o));
+        const V4 l = __builtin_bit_cast(V4, *(const uint2*)(a_lo + o));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p += (double)(acc[r][c][q] * ((float)h[q] + (float)l[q]));
+      }
+    }
+    p += __shfl_xor(p, 16);
+    p += __shfl_xor(p, 32);
+    if (lk == 0) red[wc][tl] = p;
+  }
+}
+
+template <typename T16, int WR, int WC, bool PRE = false>
 __device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __restrict__ phiv, int ld,
                                                  const float* __restrict__ pscale, const T16* __restrict__ p_hi,
                                                  const T16* __restrict__ p_lo, int np, double* __restrict__ part,
                                                  int ldp) {
-  constexpr bool PRE = AM >= 1;
-  constexpr bool AD = AM == 2;
   const float* __restrict__ phi = (const float*)phiv;
   const T16* __restrict__ a_hi = (const T16*)phiv;
   const T16* __restrict__ a_lo = a_hi + (int64_t)np * ld;
@@ -687,8 +711,7 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __r
   constexpr int TRW = kTT / WR, TCW = kTT / WC, RT = TRW / 16, CT = TCW / 16;
   constexpr int NA = 4 * kTT / NTH;        // (row, k group) pairs staged per thread and stage
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-  __shared__ u32x4 Ah[AD ? 1 : 2][AD ? 1 : 4][AD ? 1 : kTT], Al[AD ? 1 : 2][AD ? 1 : 4][AD ? 1 : kTT];
-  __shared__ u32x4 Bh[2][4][kTT], Bl[2][4][kTT];
+  __shared__ u32x4 Ah[2][4][kTT], Al[2][4][kTT], Bh[2][4][kTT], Bl[2][4][kTT];
   __shared__ double red[WC][kTT];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -709,22 +732,11 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __r
   }
   float pa[PRE ? 1 : NA * 8];
   u32x4 pah[NA], pal[NA], pbh[NA], pbl[NA];
-  // AD: fragment r of this wave = rows row0 + wr TRW + 16 r + li, k group st 4 + lk
-  u32x4 fah[RT], fal[RT], nah[RT], nal[RT];
-  auto load_frag = [&](int st, u32x4* h, u32x4* l) __attribute__((always_inline)) {
-#pragma unroll
-    for (int r = 0; r < RT; ++r) {
-      const int64_t oa = ((int64_t)(st * 4 + lk) * ld + row0 + wr * TRW + r * 16 + li) * 8;
-      h[r] = *(const u32x4*)(a_hi + oa);
-      l[r] = *(const u32x4*)(a_lo + oa);
-    }
-  };
   auto load = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       const int e = t + NTH * j, ii = e % kTT, g = e / kTT;
-      if (AD) {
-      } else if (PRE) {
+      if (PRE) {
         const int64_t oa = ((int64_t)(st * 4 + g) * ld + row0 + ii) * 8;
         pah[j] = *(const u32x4*)(a_hi + oa);
         pal[j] = *(const u32x4*)(a_lo + oa);
@@ -741,8 +753,7 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __r
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       const int e = t + NTH * j, ii = e % kTT, g = e / kTT;
-      if (AD) {
-      } else if (PRE) {
+      if (PRE) {
         Ah[buf][g][ii] = pah[j];
         Al[buf][g][ii] = pal[j];
       } else {
@@ -766,27 +777,18 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __r
   for (int r = 0; r < RT; ++r)
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[r][c] = f4{0, 0, 0, 0};
-  if (AD) load_frag(0, fah, fal);
   load(0);
   store(0);
   __syncthreads();
   for (int st = 0; st < nst; ++st) {
     const bool more = st + 1 < nst;
-    if (more) {
-      load(st + 1);
-      if (AD) load_frag(st + 1, nah, nal);
-    }
+    if (more) load(st + 1);
     const int buf = st & 1;
     V8 ah[RT], al[RT], bh[CT], bl[CT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
-      if (AD) {
-        ah[r] = __builtin_bit_cast(V8, fah[r]);
-        al[r] = __builtin_bit_cast(V8, fal[r]);
-      } else {
-        ah[r] = __builtin_bit_cast(V8, Ah[buf][lk][wr * TRW + r * 16 + li]);
-        al[r] = __builtin_bit_cast(V8, Al[buf][lk][wr * TRW + r * 16 + li]);
-      }
+      ah[r] = __builtin_bit_cast(V8, Ah[buf][lk][wr * TRW + r * 16 + li]);
+      al[r] = __builtin_bit_cast(V8, Al[buf][lk][wr * TRW + r * 16 + li]);
     }
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
@@ -797,38 +799,29 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __r
     for (int r = 0; r < RT; ++r)
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
-        acc[r][c] = Vec8<T16>::mma(al[r], bh[c], acc[r][c]);
-        acc[r][c] = Vec8<T16>::mma(ah[r], bl[c], acc[r][c]);
-        acc[r][c] = Vec8<T16>::mma(ah[r], bh[c], acc[r][c]);
-      }
-    if (more) {
-      store((st + 1) & 1);
-      if (AD) {
-#pragma unroll
-        for (int r = 0; r < RT; ++r) {
-          fah[r] = nah[r];
-          fal[r] = nal[r];
+        if (PRE) {   // transposed tiles: M first
+          acc[r][c] = Vec8<T16>::mma(bh[c], al[r], acc[r][c]);
+          acc[r][c] = Vec8<T16>::mma(bl[c], ah[r], acc[r][c]);
+          acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
+        } else {
+          acc[r][c] = Vec8<T16>::mma(al[r], bh[c], acc[r][c]);
+          acc[r][c] = Vec8<T16>::mma(ah[r], bl[c], acc[r][c]);
+          acc[r][c] = Vec8<T16>::mma(ah[r], bh[c], acc[r][c]);
         }
       }
-    }
+    if (more) store((st + 1) & 1);
     __syncthreads();
   }
+  if (PRE) {
+    presplit_rowdot<T16, RT, CT, TRW, TCW, WC>(acc, a_hi, a_lo, ld, np, row0, c0, wr, wc, li, lk, red);
+  } else {
 #pragma unroll
-  for (int r = 0; r < RT; ++r) {
-    double p4[4] = {0, 0, 0, 0};
+    for (int r = 0; r < RT; ++r) {
+      double p4[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int c = 0; c < CT; ++c) {
-      const int n = c0 + wc * TCW + c * 16 + li;
-      if (n < np) {
-        if (PRE) {
-          // element (n, t) of a plane sits at ((n / 8) ld + t) 8 + n % 8
-          const int64_t o = ((int64_t)(n >> 3) * ld + row0 + wr * TRW + r * 16 + 4 * lk) * 8 + (n & 7);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const float v = (float)a_hi[o + 8 * q] + (float)a_lo[o + 8 * q];
-            p4[q] += (double)(acc[r][c][q] * v);
-          }
-        } else {
+      for (int c = 0; c < CT; ++c) {
+        const int n = c0 + wc * TCW + c * 16 + li;
+        if (n < np) {
           const float4 v = *(const float4*)(phi + (int64_t)n * ld + row0 + wr * TRW + r * 16 + 4 * lk);
           p4[0] += (double)(acc[r][c][0] * v.x);
           p4[1] += (double)(acc[r][c][1] * v.y);
@@ -836,11 +829,11 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __r
           p4[3] += (double)(acc[r][c][3] * v.w);
         }
       }
-    }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double v = sum16(p4[q]);
-      if (li == 0) red[wc][wr * TRW + r * 16 + 4 * lk + q] = v;
+      for (int q = 0; q < 4; ++q) {
+        const double v = sum16(p4[q]);
+        if (li == 0) red[wc][wr * TRW + r * 16 + 4 * lk + q] = v;
+      }
     }
   }
   __syncthreads();
@@ -852,7 +845,7 @@ __device__ __forceinline__ void tiled_split_body(int rt, int cb, const void* __r
   }
 }
 
-template <typename T16, int WR, int WC, int AM = 0>
+template <typename T16, int WR, int WC, bool PRE = false>
 __global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const void* __restrict__ phi, int ld,
                                                                       const float* __restrict__ pscale,
                                                                       const T16* __restrict__ p_hi,
@@ -860,43 +853,36 @@ __global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const voi
                                                                       int ncb, double* __restrict__ part) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  tiled_split_body<T16, WR, WC, AM>(rt, cb, phi, ld, pscale, p_hi, p_lo, np, part, ld);
+  tiled_split_body<T16, WR, WC, PRE>(rt, cb, phi, ld, pscale, p_hi, p_lo, np, part, ld);
 }
 
 // Tiled split contraction on pre-split basis values with both operands
 // fetched by LDS DMA (global_load_lds_dwordx4: each lane's 16 bytes land at
 // M0 + 16 lane, so a wave fills 1 KB of LDS without touching registers) into a
-// ring of NB stages of 32 k; stage st + NB - 1 is requested while stage st is
-// multiplied, so NB - 1 stages of loads are in flight behind the MFMAs.  The
-// ring slot written at stage st was last read at stage st - 1, which every wave
-// has finished once it passes stage st's barrier.  The loads are issued from
-// inline assembly (M0 holds the LDS address) so that the compiler does not wait
-// for all of them before each LDS read; the explicit s_waitcnt vmcnt before the
-// barrier waits for exactly the oldest stage.  No other code in this kernel uses
-// M0.  Tiles, partial sums and the epilogue are those of tiled_split_body (AM 1).
+// two-stage ring of 32 k: stage st + 1 is requested right after the barrier
+// that opens stage st, so its loads run behind stage st's MFMAs while the
+// workgroup's registers stay free (two workgroups share a CU).  The ring slot
+// written at stage st was last read at stage st - 1, which every wave has
+// finished once it passes stage st's barrier.  The loads are issued from inline
+// assembly (M0 holds the LDS address) so that the compiler does not wait for
+// them before each LDS read; the explicit s_waitcnt vmcnt before the barrier
+// waits for them instead.  No other code in this kernel uses M0.  Tiles, the
+// transposed MFMA order and the row-dot are those of tiled_split_body<PRE>.
 __device__ __forceinline__ void lds_dma16(const void* gsrc, const void* lds_wave_base) {
   const unsigned l = __builtin_amdgcn_readfirstlane(
       (unsigned)(uintptr_t)(__attribute__((address_space(3))) const void*)lds_wave_base);
   asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(l), "v"(gsrc) : "memory");
 }
 
-// wait until at most 4 k of this thread's DMA loads are outstanding (4 per stage)
-__device__ __forceinline__ void dma_wait_stages(int k) {
-  if (k >= 2) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
-  else if (k == 1) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
-  else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-}
-
-template <typename T16, int NB>
+template <typename T16>
 __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __restrict__ a_hi, int ld,
                                                const T16* __restrict__ p_hi, const T16* __restrict__ p_lo, int np,
                                                double* __restrict__ part, int ldp) {
-  static_assert(NB >= 2 && NB <= 4, "ring of 2..4 stages");
   typedef typename Vec8<T16>::type V8;
-  constexpr int WR = 4, WC = 2, NTH = 512, TRW = kTT / WR, TCW = kTT / WC, RT = TRW / 16, CT = TCW / 16;
+  constexpr int WR = 4, WC = 2, TRW = kTT / WR, TCW = kTT / WC, RT = TRW / 16, CT = TCW / 16;
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   // ring slot: 4 operand planes (A hi, A lo, B hi, B lo) of [4 k groups][128] 16-byte words
-  __shared__ u32x4 ring[NB][4][4][kTT];
+  __shared__ u32x4 ring[2][4][4][kTT];
   __shared__ double red[WC][kTT];
   const T16* __restrict__ a_lo = a_hi + (int64_t)np * ld;
   const int t = threadIdx.x, lane = t & 63;
@@ -909,7 +895,7 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
   const int g = t / kTT, ii = t % kTT;
   const int bcol = min(c0 + ii, np - 1);   // columns past the matrix are never used
   auto issue = [&](int st) __attribute__((always_inline)) {
-    const int slot = st % NB;
+    const int slot = st & 1;
     const int64_t oa = ((int64_t)(st * 4 + g) * ld + row0 + ii) * 8;
     const int64_t ob = ((int64_t)(st * 4 + g) * np + bcol) * 8;
     lds_dma16(a_hi + oa, &ring[slot][0][0][0] + 64 * wave);
@@ -922,14 +908,12 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
   for (int r = 0; r < RT; ++r)
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[r][c] = f4{0, 0, 0, 0};
-#pragma unroll
-  for (int st = 0; st < NB - 1; ++st)
-    if (st < nst) issue(st);
+  issue(0);
   for (int st = 0; st < nst; ++st) {
-    dma_wait_stages(min(NB - 2, nst - 1 - st));   // stages st+1 .. st+NB-2 may stay in flight
-    __syncthreads();
-    if (st + NB - 1 < nst) issue(st + NB - 1);
-    const int slot = st % NB;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this thread's loads of stage st
+    __syncthreads();                                    // everyone's loads of st; everyone done with st - 1
+    if (st + 1 < nst) issue(st + 1);
+    const int slot = st & 1;
     V8 ah[RT], al[RT], bh[CT], bl[CT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
@@ -944,46 +928,24 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
 #pragma unroll
     for (int r = 0; r < RT; ++r)
 #pragma unroll
-      for (int c = 0; c < CT; ++c) {
-        acc[r][c] = Vec8<T16>::mma(al[r], bh[c], acc[r][c]);
-        acc[r][c] = Vec8<T16>::mma(ah[r], bl[c], acc[r][c]);
-        acc[r][c] = Vec8<T16>::mma(ah[r], bh[c], acc[r][c]);
+      for (int c = 0; c < CT; ++c) {   // transposed tiles: M first
+        acc[r][c] = Vec8<T16>::mma(bh[c], al[r], acc[r][c]);
+        acc[r][c] = Vec8<T16>::mma(bl[c], ah[r], acc[r][c]);
+        acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
       }
   }
-#pragma unroll
-  for (int r = 0; r < RT; ++r) {
-    double p4[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-      const int n = c0 + wc * TCW + c * 16 + li;
-      if (n < np) {
-        // element (n, t) of a plane sits at ((n / 8) ld + t) 8 + n % 8
-        const int64_t o = ((int64_t)(n >> 3) * ld + row0 + wr * TRW + r * 16 + 4 * lk) * 8 + (n & 7);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float v = (float)a_hi[o + 8 * q] + (float)a_lo[o + 8 * q];
-          p4[q] += (double)(acc[r][c][q] * v);
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double v = sum16(p4[q]);
-      if (li == 0) red[wc][wr * TRW + r * 16 + 4 * lk + q] = v;
-    }
-  }
+  presplit_rowdot<T16, RT, CT, TRW, TCW, WC>(acc, a_hi, a_lo, ld, np, row0, c0, wr, wc, li, lk, red);
   __syncthreads();
   if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
 }
 
-template <typename T16, int NB>
+template <typename T16>
 __global__ __launch_bounds__(512) void k_contract_split_dma(const void* __restrict__ phi, int ld,
-                                                            const T16* __restrict__ p_hi,
-                                                            const T16* __restrict__ p_lo, int np, int nrt, int ncb,
-                                                            double* __restrict__ part) {
+                                                            const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
+                                                            int np, int nrt, int ncb, double* __restrict__ part) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  tiled_dma_body<T16, NB>(rt, cb, (const T16*)phi, ld, p_hi, p_lo, np, part, ld);
+  tiled_dma_body<T16>(rt, cb, (const T16*)phi, ld, p_hi, p_lo, np, part, ld);
 }
 
 // Variant 3: tiles of 128 points x 64 columns with waves stacked 8 x 1 (wave w
@@ -1467,33 +1429,32 @@ float split_phi_scale(const qm_mol* m) {
 }
 
 // Split variants of the contraction: 0 / 2 tiles of 8 (4 x 2) / 4 (2 x 2) waves on FP32
-// basis values; 1 / 3 the same on pre-split values staged through LDS; 4 / 5 on
-// pre-split values loaded straight into registers; 6 / 7 / 8 on pre-split values
-// fetched by LDS DMA into a ring of 2 / 3 / 4 stages; 64 / 128 all-columns kernels.
-inline bool split_tiled(int v) { return v >= 0 && v <= 8; }
-inline bool split_pre(int v) { return v >= 1 && v <= 8 && v != 2; }
+// basis values; 1 tiles of 8 waves on pre-split values staged through registers;
+// 3 the same fed by LDS DMA; 64 / 128 all-columns kernels.
+inline bool split_tiled(int v) { return v >= 0 && v <= 3; }
+inline bool split_pre(int v) { return v == 1 || v == 3; }
 
 template <typename T16>
 void launch_split_tiles(hipStream_t s, int v, int blocks, const void* phi, int ld, const float* pscale,
                         const T16* Ph, const T16* Pl, int np, int nrt, int ncb, double* part) {
-#define QM_TILES(WR, WC, AM)                                                                              \
-  hipLaunchKernelGGL((k_contract_split_tiles<T16, WR, WC, AM>), dim3(blocks), dim3(64 * WR * WC), 0, s, phi, \
-                     ld, pscale, Ph, Pl, np, nrt, ncb, part)
-#define QM_DMA(NB) \
-  hipLaunchKernelGGL((k_contract_split_dma<T16, NB>), dim3(blocks), dim3(512), 0, s, phi, ld, Ph, Pl, np, nrt, ncb, part)
   switch (v) {
-    case 1: QM_TILES(4, 2, 1); break;
-    case 2: QM_TILES(2, 2, 0); break;
-    case 3: QM_TILES(2, 2, 1); break;
-    case 4: QM_TILES(4, 2, 2); break;
-    case 5: QM_TILES(2, 2, 2); break;
-    case 6: QM_DMA(2); break;
-    case 7: QM_DMA(3); break;
-    case 8: QM_DMA(4); break;
-    default: QM_TILES(4, 2, 0); break;
+    case 1:
+      hipLaunchKernelGGL((k_contract_split_tiles<T16, 4, 2, true>), dim3(blocks), dim3(512), 0, s, phi, ld, pscale,
+                         Ph, Pl, np, nrt, ncb, part);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_contract_split_tiles<T16, 2, 2>), dim3(blocks), dim3(256), 0, s, phi, ld, pscale, Ph, Pl,
+                         np, nrt, ncb, part);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_contract_split_dma<T16>, dim3(blocks), dim3(512), 0, s, phi, ld, Ph, Pl, np, nrt, ncb,
+                         part);
+      break;
+    default:
+      hipLaunchKernelGGL((k_contract_split_tiles<T16, 4, 2>), dim3(blocks), dim3(512), 0, s, phi, ld, pscale, Ph, Pl,
+                         np, nrt, ncb, part);
+      break;
   }
-#undef QM_TILES
-#undef QM_DMA
 }
 
 template <typename T16, typename TO>
@@ -1653,7 +1614,7 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
   if (rc) return rc;
   if (knob == QM_TUNE_SPLIT_VARIANT) {
     QM_REQUIRE(ctx, split_tiled(value) || value == 64 || value == 128, QM_EINVAL,
-               "tune: split variant must be 0..5 (tiles) or 64 / 128 (rows per workgroup)");
+               "tune: split variant must be 0..3 (tiles) or 64 / 128 (rows per workgroup)");
     ctx->split_tm = value;
     return QM_OK;
   }
@@ -1702,7 +1663,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   void* out = nullptr;
   if ((rc = qm_ws(ctx, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
-  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 4, 5, 6, 7, 8, 64, 128} : std::vector<int>{0, 1, 2, 3};
+  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 64, 128} : std::vector<int>{0, 1, 2, 3};
   int& knob = split ? ctx->split_tm : ctx->f64_variant;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));
@@ -1885,42 +1846,31 @@ __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __re
   tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 
-template <typename T16, int WR, int WC, int AM = 0>
+template <typename T16, int WR, int WC, bool PRE = false>
 __global__ __launch_bounds__(64 * WR * WC) void kb_contract_split_tiles(const BChunk* __restrict__ c, int n) {
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
-  tiled_split_body<T16, WR, WC, AM>(rt, cb, q.phi, q.ld, q.pscale, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad,
+  tiled_split_body<T16, WR, WC, PRE>(rt, cb, q.phi, q.ld, q.pscale, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad,
                                      q.part, q.ld);
 }
 
-template <typename T16, int NB>
+template <typename T16>
 __global__ __launch_bounds__(512) void kb_contract_split_dma(const BChunk* __restrict__ c, int n) {
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
-  tiled_dma_body<T16, NB>(rt, cb, (const T16*)q.phi, q.ld, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad, q.part,
-                          q.ld);
+  tiled_dma_body<T16>(rt, cb, (const T16*)q.phi, q.ld, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad, q.part, q.ld);
 }
 
 template <typename T16>
 void launch_batch_split_tiles(hipStream_t s, int v, int blocks, const BChunk* c, int n) {
-#define QM_BTILES(WR, WC, AM) \
-  hipLaunchKernelGGL((kb_contract_split_tiles<T16, WR, WC, AM>), dim3(blocks), dim3(64 * WR * WC), 0, s, c, n)
-#define QM_BDMA(NB) hipLaunchKernelGGL((kb_contract_split_dma<T16, NB>), dim3(blocks), dim3(512), 0, s, c, n)
   switch (v) {
-    case 1: QM_BTILES(4, 2, 1); break;
-    case 2: QM_BTILES(2, 2, 0); break;
-    case 3: QM_BTILES(2, 2, 1); break;
-    case 4: QM_BTILES(4, 2, 2); break;
-    case 5: QM_BTILES(2, 2, 2); break;
-    case 6: QM_BDMA(2); break;
-    case 7: QM_BDMA(3); break;
-    case 8: QM_BDMA(4); break;
-    default: QM_BTILES(4, 2, 0); break;
+    case 1: hipLaunchKernelGGL((kb_contract_split_tiles<T16, 4, 2, true>), dim3(blocks), dim3(512), 0, s, c, n); break;
+    case 2: hipLaunchKernelGGL((kb_contract_split_tiles<T16, 2, 2>), dim3(blocks), dim3(256), 0, s, c, n); break;
+    case 3: hipLaunchKernelGGL(kb_contract_split_dma<T16>, dim3(blocks), dim3(512), 0, s, c, n); break;
+    default: hipLaunchKernelGGL((kb_contract_split_tiles<T16, 4, 2>), dim3(blocks), dim3(512), 0, s, c, n); break;
   }
-#undef QM_BTILES
-#undef QM_BDMA
 }
 
 // same sums as k_tiles_finish; one thread per point, blocks numbered like the basis-value launch

@@ -105,7 +105,7 @@ def test_density_mixed(qm, small, prec, tol):
 
 @pytest.mark.parametrize("prec,tol", [("f16", 2e-6), ("bf16", 1e-4)])
 @pytest.mark.parametrize("subgrid", [1, 2])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_density_mixed_tiled(qm, small, prec, tol, subgrid, variant):
     """The 128 x 128 tiled split contraction (partial sums per column block) against the oracle."""
     from oracle import density as od
@@ -349,7 +349,7 @@ def test_density_fp64_large_basis(qm, large, variant):
     assert q.raw_total == pytest.approx(raw, rel=1e-10)
 
 
-@pytest.mark.parametrize("rows", [0, 1, 2, 3, 4, 5, 6, 7, 8, 64, 128])
+@pytest.mark.parametrize("rows", [0, 1, 2, 3, 64, 128])
 def test_density_f16_large_basis_tiles(qm, large, rows):
     from oracle import density as od
     from qmtools import autotune
@@ -377,7 +377,7 @@ def test_autotune_cache(qm, small, tmp_path):
     _say("autotune", grid.shape)
     old = autotune.get("split_variant")
     e = autotune.autotune_density(mol, grid, "mixed", path=path)
-    assert e["split_variant"] in (0, 1, 2, 3, 4, 5, 6, 7, 8, 64, 128) and e["ms"] > 0
+    assert e["split_variant"] in (0, 1, 2, 3, 64, 128) and e["ms"] > 0
     table = json.load(open(path))
     assert autotune.key_for(mol, "mixed") in table
     autotune.set("split_variant", 64)

@@ -37,11 +37,14 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per contraction launch measured with rocprofv3 FETCH_SIZE (x2 calibration) and
 # WRITE_SIZE (x1) for this workload; source: profiles/config2_pmc_traffic.md
 # (precision, nbf_pad, variant): (points per launch, HBM bytes per launch)
-MEASURED_TRAFFIC = {("fp64", 640, "tiles"): (838784, 6028944 * 1024 * 2 + 32765 * 1024),
+MEASURED_TRAFFIC = {("fp64", 640, "tiles_dma"): (838784, 6558252 * 1024 * 2 + 32765 * 1024),
+                    ("fp64", 640, "tiles"): (838784, 6138183 * 1024 * 2 + 32765 * 1024),
                     ("fp64", 640, "rows"): (838784, 3870778 * 1024 * 2 + 65536 * 1024),
                     ("fp64", 640, "lds"): (838784, 4444220 * 1024 * 2 + 190037 * 1024),
                     ("fp64", 640, "direct"): (838784, 7883104 * 1024 * 2 + 6553 * 1024),
-                    ("mixed", 640, 0): (1677696, 2396583 * 1024 * 2 + 65535 * 1024),
+                    ("mixed", 640, 3): (1677696, 2191921 * 1024 * 2 + 65535 * 1024),
+                    ("mixed", 640, 1): (1677696, 2246457 * 1024 * 2 + 65535 * 1024),
+                    ("mixed", 640, 0): (1677696, 2397406 * 1024 * 2 + 65535 * 1024),
                     ("mixed", 640, 2): (1677696, 2526254 * 1024 * 2 + 65535 * 1024),
                     ("mixed", 640, 64): (1677696, 3798200 * 1024 * 2 + 530834 * 1024),
                     ("mixed", 640, 128): (1677824, 6713930 * 1024 * 2 + 8984396 * 1024)}
@@ -433,7 +436,8 @@ def main():
     achieved = flops_per_launch * mult / (avg_ms * 1e-3) / 1e12
     dense_equiv = (2.0 * nbf * nbf + 2.0 * nbf) * launch_pts * mult / (avg_ms * 1e-3) / 1e12
     peak = FP64_MFMA_PEAK_TFLOPS if args.precision == "fp64" else F16_MFMA_PEAK_TFLOPS
-    variant = ({0: "lds", 1: "direct", 2: "tiles", 3: "rows"}[tune["fp64"].get("f64_variant", 0)] if args.precision == "fp64"
+    variant = ({0: "lds", 1: "direct", 2: "tiles", 3: "rows", 4: "tiles_dma"}[tune["fp64"].get("f64_variant", 0)]
+               if args.precision == "fp64"
                else tune["mixed"].get("split_variant"))
     meas = MEASURED_TRAFFIC.get((args.precision, (nbf + 63) // 64 * 64, variant))
     traffic = None if meas is None else meas[1] / meas[0] * launch_pts

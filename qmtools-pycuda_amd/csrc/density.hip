@@ -572,6 +572,42 @@ __device__ __forceinline__ bool tile_coords(int b, int nrt, int ncb, int& rt, in
 
 inline int tile_grid(int nrt, int ncb) { return (nrt + 7) / 8 * 8 * ncb; }
 
+// LDS DMA of 16 bytes per lane: global_load_lds_dwordx4 writes lane i's 16 bytes
+// at M0 + 16 i, so a wave fills 1 KB of LDS starting at lds_wave_base (uniform)
+// without passing through registers.  Issued from inline assembly so that the
+// compiler does not add a wait for every outstanding load before each LDS read;
+// callers wait with an explicit s_waitcnt vmcnt before the barrier that
+// publishes the data.  No other code in these kernels uses M0.
+__device__ __forceinline__ void lds_dma16(const void* gsrc, const void* lds_wave_base) {
+  const unsigned l = __builtin_amdgcn_readfirstlane(
+      (unsigned)(uintptr_t)(__attribute__((address_space(3))) const void*)lds_wave_base);
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(l), "v"(gsrc) : "memory");
+}
+
+// Row-dot of the transposed FP64 tiles (M was the first MFMA operand): lane
+// (li, lk) register q of tile (r, c) is X[t = li][n = lk + 4 q]; 16 lanes read 16
+// consecutive points of one function (128 contiguous bytes); the 4 lane groups
+// that share a point are added by two butterflies.
+__device__ __forceinline__ void f64_rowdot(const d4 (&acc)[2][4], const double* __restrict__ phi, int ld, int np,
+                                           int row0, int c0, int wr, int wc, int li, int lk, double (*red)[kTT]) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int tl = wr * 32 + r * 16 + li;
+    double p = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int n = c0 + wc * 64 + c * 16 + lk + 4 * q;
+        if (n < np) p += acc[r][c][q] * phi[(int64_t)n * ld + row0 + tl];
+      }
+    }
+    p += __shfl_xor(p, 16);
+    p += __shfl_xor(p, 32);
+    if (lk == 0) red[wc][tl] = p;
+  }
+}
+
 __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __restrict__ phi, int ld,
                                                const double* __restrict__ M, int np, double* __restrict__ part,
                                                int ldp) {
@@ -629,29 +665,84 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+        for (int c = 0; c < 4; ++c)   // M first: tile (r, c) holds X^T, lane (li, lk) register q = X[t = li][n = lk + 4 q]
+          acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c], a[r], acc[r][c], 0, 0, 0);
     }
     if (more) store((st + 1) & 1);
     __syncthreads();
   }
+  f64_rowdot(acc, phi, ld, np, row0, c0, wr, wc, li, lk, red);
+  __syncthreads();
+  if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
+}
+
+// Tiled FP64 contraction fed by LDS DMA (variant 4): tiles, waves, MFMA order
+// and row-dot of tiled_f64_body; both operands go global -> LDS by
+// global_load_lds_dwordx4 (lane i of a wave brings points / columns 2i, 2i+1 of
+// one k row, so a wave fills one padded LDS row) into a two-stage ring, issued
+// right after the barrier that opens the previous stage.  The loads come from
+// inline assembly (lds_dma16), the wait before each barrier is explicit.
+__device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double* __restrict__ phi, int ld,
+                                                   const double* __restrict__ M, int np, double* __restrict__ part,
+                                                   int ldp) {
+  __shared__ __align__(16) double ring[2][2][kTKS][kTT + kTPad];
+  __shared__ double red[2][kTT];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  // waves w and w + 4 (one SIMD when waves are dealt round-robin) take the two column halves,
+  // so the MFMAs skipped in the diagonal block below are spread evenly over the SIMDs
+  const int wr = wave & 3, wc = wave >> 2;
+  const int li = lane & 15, lk = lane >> 4;
+  const int row0 = rt * kTT, c0 = cb * kTT;
+  const int nst = min(np, c0 + kTT) / kTKS;
+  // wave w brings k rows 4 (w % 4) .. 4 (w % 4) + 3 of operand w / 4 (0: basis values, 1: M)
+  const int op = wave >> 2, k0 = 4 * (wave & 3);
+  const double* src = op == 0 ? phi + row0 + 2 * lane : M + min(c0 + 2 * lane, np - 2);   // past np: never used
+  const int64_t lds_ld = op == 0 ? ld : np;
+  auto issue = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    double p4[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + j;
+      lds_dma16(src + (int64_t)(st * kTKS + k) * lds_ld, &ring[st & 1][op][k][0]);
+    }
+  };
+  d4 acc[2][4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int n = c0 + wc * 64 + c * 16 + li;
-      if (n < np) {
-        const double* ph = phi + (int64_t)n * ld + row0 + wr * 32 + r * 16 + lk;
+  for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) p4[q] += acc[r][c][q] * ph[4 * q];
+    for (int c = 0; c < 4; ++c) acc[r][c] = d4{0, 0, 0, 0};
+  issue(0);
+  for (int st = 0; st < nst; ++st) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this thread's loads of stage st
+    __syncthreads();                                    // everyone's loads of st; everyone done with st - 1
+    if (st + 1 < nst) issue(st + 1);
+    const int buf = st & 1;
+    // M is zero below its diagonal: in the block's diagonal stages (j = st - c0 / 16 >= 0) tile
+    // column c (columns c0 + 64 wc + 16 c ..) needs this stage only while 16 j <= 64 wc + 16 c,
+    // i.e. c >= j - 4 wc.  The first needed column is uniform per wave; each case is its own
+    // straight-line body so the accumulators stay in place.
+    auto body = [&](auto first) __attribute__((always_inline)) {
+      constexpr int cs = decltype(first)::value;
+#pragma unroll
+      for (int ks = 0; ks < kTKS; ks += 4) {
+        double a[2], b[4];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) a[r] = ring[buf][0][ks + lk][wr * 32 + r * 16 + li];
+#pragma unroll
+        for (int c = cs; c < 4; ++c) b[c] = ring[buf][1][ks + lk][wc * 64 + c * 16 + li];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int c = cs; c < 4; ++c)
+            acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c], a[r], acc[r][c], 0, 0, 0);
       }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double v = sum16(p4[q]);
-      if (li == 0) red[wc][wr * 32 + r * 16 + 4 * q + lk] = v;
-    }
+    };
+    const int cs = min(max(st - c0 / kTKS - 4 * wc, 0), 4);
+    // (a skipped tile would only add products with M's zeros, so cases may round cs down)
+    if (cs < 2) body(std::integral_constant<int, 0>());
+    else if (cs < 4) body(std::integral_constant<int, 2>());
   }
+  f64_rowdot(acc, phi, ld, np, row0, c0, wr, wc, li, lk, red);
   __syncthreads();
   if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
 }
@@ -868,11 +959,6 @@ __global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const voi
 // them before each LDS read; the explicit s_waitcnt vmcnt before the barrier
 // waits for them instead.  No other code in this kernel uses M0.  Tiles, the
 // transposed MFMA order and the row-dot are those of tiled_split_body<PRE>.
-__device__ __forceinline__ void lds_dma16(const void* gsrc, const void* lds_wave_base) {
-  const unsigned l = __builtin_amdgcn_readfirstlane(
-      (unsigned)(uintptr_t)(__attribute__((address_space(3))) const void*)lds_wave_base);
-  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(l), "v"(gsrc) : "memory");
-}
 
 template <typename T16>
 __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __restrict__ a_hi, int ld,
@@ -887,7 +973,7 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
   const T16* __restrict__ a_lo = a_hi + (int64_t)np * ld;
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int wr = wave / WC, wc = wave % WC;
+  const int wr = wave & 3, wc = wave >> 2;   // waves w, w + 4 (one SIMD) take the two column halves
   const int li = lane & 15, lk = lane >> 4;
   const int row0 = rt * kTT, c0 = cb * kTT;
   const int nst = min(np, c0 + kTT) / 32;
@@ -914,25 +1000,34 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
     __syncthreads();                                    // everyone's loads of st; everyone done with st - 1
     if (st + 1 < nst) issue(st + 1);
     const int slot = st & 1;
-    V8 ah[RT], al[RT], bh[CT], bl[CT];
+    // M is zero below its diagonal: in the block's diagonal stages (j = st - c0 / 32 >= 0) tile
+    // column c (columns c0 + 64 wc + 16 c ..) needs this stage only while 32 j < 64 wc + 16 c + 16,
+    // i.e. c >= 2 j - 4 wc; a tile computed anyway only adds products with M's zeros.
+    auto body = [&](auto first) __attribute__((always_inline)) {
+      constexpr int cs = decltype(first)::value;
+      V8 ah[RT], al[RT], bh[CT], bl[CT];
 #pragma unroll
-    for (int r = 0; r < RT; ++r) {
-      ah[r] = __builtin_bit_cast(V8, ring[slot][0][lk][wr * TRW + r * 16 + li]);
-      al[r] = __builtin_bit_cast(V8, ring[slot][1][lk][wr * TRW + r * 16 + li]);
-    }
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-      bh[c] = __builtin_bit_cast(V8, ring[slot][2][lk][wc * TCW + c * 16 + li]);
-      bl[c] = __builtin_bit_cast(V8, ring[slot][3][lk][wc * TCW + c * 16 + li]);
-    }
-#pragma unroll
-    for (int r = 0; r < RT; ++r)
-#pragma unroll
-      for (int c = 0; c < CT; ++c) {   // transposed tiles: M first
-        acc[r][c] = Vec8<T16>::mma(bh[c], al[r], acc[r][c]);
-        acc[r][c] = Vec8<T16>::mma(bl[c], ah[r], acc[r][c]);
-        acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
+      for (int r = 0; r < RT; ++r) {
+        ah[r] = __builtin_bit_cast(V8, ring[slot][0][lk][wr * TRW + r * 16 + li]);
+        al[r] = __builtin_bit_cast(V8, ring[slot][1][lk][wr * TRW + r * 16 + li]);
       }
+#pragma unroll
+      for (int c = cs; c < CT; ++c) {
+        bh[c] = __builtin_bit_cast(V8, ring[slot][2][lk][wc * TCW + c * 16 + li]);
+        bl[c] = __builtin_bit_cast(V8, ring[slot][3][lk][wc * TCW + c * 16 + li]);
+      }
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int c = cs; c < CT; ++c) {   // transposed tiles: M first
+          acc[r][c] = Vec8<T16>::mma(bh[c], al[r], acc[r][c]);
+          acc[r][c] = Vec8<T16>::mma(bl[c], ah[r], acc[r][c]);
+          acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
+        }
+    };
+    const int cs = min(max(2 * (st - c0 / 32) - 4 * wc, 0), 4);
+    if (cs < 2) body(std::integral_constant<int, 0>());
+    else if (cs < 4) body(std::integral_constant<int, 2>());
   }
   presplit_rowdot<T16, RT, CT, TRW, TCW, WC>(acc, a_hi, a_lo, ld, np, row0, c0, wr, wc, li, lk, red);
   __syncthreads();
@@ -1047,12 +1142,14 @@ __global__ __launch_bounds__(kTWG) void k_contract_f64_rows(const double* __rest
   tiled_f64_rows_body(rt, cb, phi, ld, M, np, part, ld);
 }
 
+template <bool DMA>
 __global__ __launch_bounds__(kTWG) void k_contract_f64_tiles(const double* __restrict__ phi, int ld,
                                                              const double* __restrict__ M, int np, int nrt, int ncb,
                                                              double* __restrict__ part) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  tiled_f64_body(rt, cb, phi, ld, M, np, part, ld);
+  if (DMA) tiled_f64_dma_body(rt, cb, phi, ld, M, np, part, ld);
+  else tiled_f64_body(rt, cb, phi, ld, M, np, part, ld);
 }
 
 // rho_t = scale * sum_cb part[cb][t] (fixed order), optionally / pscale_t
@@ -1355,7 +1452,7 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
     rc = qm_ws(ctx, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
     if (rc) return rc;
   }
-  const bool tiled = ctx->f64_variant == 2 || ctx->f64_variant == 3;
+  const bool tiled = ctx->f64_variant >= 2;   // 2 tiles, 3 rows, 4 tiles fed by LDS DMA
   const int ncb = ctx->f64_variant == 3 ? np / kRN : (np + kTT - 1) / kTT;
   double* part = nullptr;
   if (tiled && (rc = qm_ws(ctx, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
@@ -1378,9 +1475,12 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
       if (ctx->f64_variant == 3)
         hipLaunchKernelGGL(k_contract_f64_rows, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const double*)phi, ld,
                            (const double*)mol->M64, np, nrt, ncb, part);
+      else if (ctx->f64_variant == 4)
+        hipLaunchKernelGGL(k_contract_f64_tiles<true>, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s,
+                           (const double*)phi, ld, (const double*)mol->M64, np, nrt, ncb, part);
       else
-        hipLaunchKernelGGL(k_contract_f64_tiles, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const double*)phi, ld,
-                           (const double*)mol->M64, np, nrt, ncb, part);
+        hipLaunchKernelGGL(k_contract_f64_tiles<false>, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s,
+                           (const double*)phi, ld, (const double*)mol->M64, np, nrt, ncb, part);
       QM_CHECK_LAUNCH(ctx);
       if (nsub3 == 1) {
         hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncb,
@@ -1619,7 +1719,7 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
     return QM_OK;
   }
   if (knob == QM_TUNE_F64_VARIANT) {
-    QM_REQUIRE(ctx, value >= 0 && value <= 3, QM_EINVAL, "tune: f64 variant must be 0, 1, 2 or 3");
+    QM_REQUIRE(ctx, value >= 0 && value <= 4, QM_EINVAL, "tune: f64 variant must be 0..4");
     ctx->f64_variant = value;
     return QM_OK;
   }
@@ -1663,7 +1763,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   void* out = nullptr;
   if ((rc = qm_ws(ctx, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
-  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 64, 128} : std::vector<int>{0, 1, 2, 3};
+  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 64, 128} : std::vector<int>{0, 1, 2, 3, 4};
   int& knob = split ? ctx->split_tm : ctx->f64_variant;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));
@@ -1839,11 +1939,13 @@ __global__ __launch_bounds__(kTWG) void kb_contract_f64_rows(const BChunk* __res
   tiled_f64_rows_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 
+template <bool DMA>
 __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __restrict__ c, int n) {
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
-  tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
+  if (DMA) tiled_f64_dma_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
+  else tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 
 template <typename T16, int WR, int WC, bool PRE = false>
@@ -1911,7 +2013,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   const bool f64 = precision == QM_PREC_FP64;
   const size_t elt = f64 ? sizeof(double) : sizeof(float);
   const int tm = f64 ? 64 : ctx->split_tm;
-  const bool tiled = f64 ? (ctx->f64_variant == 2 || ctx->f64_variant == 3) : split_tiled(ctx->split_tm);
+  const bool tiled = f64 ? ctx->f64_variant >= 2 : split_tiled(ctx->split_tm);
   // basis values written scaled and split (same bytes as FP32)
   const bool pre = !f64 && split_pre(ctx->split_tm);
   // chunks per molecule, packed into rounds of at most kPhiBudget bytes of basis values
@@ -2045,8 +2147,10 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       if (tiled) {
         if (f64 && ctx->f64_variant == 3)
           hipLaunchKernelGGL(kb_contract_f64_rows, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+        else if (f64 && ctx->f64_variant == 4)
+          hipLaunchKernelGGL(kb_contract_f64_tiles<true>, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
         else if (f64)
-          hipLaunchKernelGGL(kb_contract_f64_tiles, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
+          hipLaunchKernelGGL(kb_contract_f64_tiles<false>, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
         else if (precision == QM_PREC_F16X3)
           launch_batch_split_tiles<_Float16>(s, ctx->split_tm, blocks, gc, gn);
         else

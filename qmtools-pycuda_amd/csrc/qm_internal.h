@@ -101,7 +101,7 @@ struct qm_ctx {
   hipStream_t graph_stream = nullptr;   // non-blocking stream for graph capture (default priority)
   std::vector<LiveGraph> graphs;        // graphs whose replays may still run
   int split_tm = 3;                     // split contraction variant (0..3 tiled, 64 / 128 all-columns)
-  int f64_variant = 2;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled
+  int f64_variant = 4;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled, 3 rows, 4 tiled + DMA
   int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)
   std::vector<ProfRec> recs;
 };

@@ -24,7 +24,7 @@ def items():
 
 
 @pytest.mark.parametrize("precision,knob,variant", [("fp64", "f64_variant", 2), ("fp64", "f64_variant", 0),
-                                                    ("fp64", "f64_variant", 3),
+                                                    ("fp64", "f64_variant", 3), ("fp64", "f64_variant", 4),
                                                     ("mixed", "split_variant", 0), ("mixed", "split_variant", 1),
                                                     ("mixed", "split_variant", 2), ("mixed", "split_variant", 3),
                                                     ("mixed", "split_variant", 128)])

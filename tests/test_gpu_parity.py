@@ -59,7 +59,7 @@ def _say(*a):
 
 
 # ---------------------------------------------------------------- density
-@pytest.mark.parametrize("subgrid,variant", [(1, 0), (2, 0), (1, 1), (2, 1), (1, 2), (2, 2), (1, 3), (2, 3)])
+@pytest.mark.parametrize("subgrid,variant", [(1, 0), (2, 0), (1, 1), (2, 1), (1, 2), (2, 2), (1, 3), (2, 3), (1, 4), (2, 4)])
 def test_density_fp64_synthetic(qm, small, subgrid, variant):
     from oracle import density as od
     from qmtools import autotune
@@ -331,7 +331,7 @@ def large(request, qm, obasis):
     return mol, grid, _system(mol, obasis)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_density_fp64_large_basis(qm, large, variant):
     from oracle import density as od
     from qmtools import autotune

@@ -19,8 +19,8 @@
 // powers of T = gamma |PC|^2 and lose precision there).
 //
 // Boys function F_n(T): for T < 40, Taylor expansion (8 terms) around the
-// nearest point of a 0.1-spaced table of F_0..F_12 (computed on the host from
-// the all-positive series e^-T sum (2T)^k / ((2n+1)...(2n+2k+1))), then
+// nearest point of a 0.1-spaced table of F_0..F_11 and e^-T (computed on the host
+// from the all-positive series e^-T sum (2T)^k / ((2n+1)...(2n+2k+1))), then
 // downward recursion F_{n-1} = (2T F_n + e^-T)/(2n-1); for T >= 40,
 // F_0 = sqrt(pi/T)/2 and upward recursion (e^-40 is below FP64 resolution of F).
 #include <algorithm>
@@ -33,7 +33,7 @@ namespace {
 
 constexpr int kLmax = 4;                 // La + Lb <= 4
 constexpr int kNTUV = 35;                // (t,u,v) with t+u+v <= 4
-constexpr int kBoysN = 13;               // F_0 .. F_12 tabulated
+constexpr int kBoysN = 13;               // F_0 .. F_11 (up to L + 7 for L <= 4), then e^-T
 constexpr int kBoysPts = 401;            // T = 0, 0.1, ..., 40
 constexpr double kBoysStep = 0.1;
 constexpr double kBoysTmax = 40.0;
@@ -51,24 +51,34 @@ struct HDist {
 };
 
 // ------------------------------------------------------------------ device
+// 1 / k for the Taylor factors and 1 / (2n - 1) for the recursion: multiplications by
+// rounded reciprocals instead of FP64 divisions (each a ~10-instruction sequence)
+__device__ constexpr double kInvInt[10] = {0.0, 1.0, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 5, 1.0 / 6, 1.0 / 7, 1.0 / 8, 1.0 / 9};
+
 template <int L>
 __device__ __forceinline__ void boys(double T, const double* __restrict__ tab, double* F) {
   if (T < kBoysTmax) {
     const int i = (int)(T * (1.0 / kBoysStep) + 0.5);
-    const double d = i * kBoysStep - T;       // T = Ti - d
+    const double d = i * kBoysStep - T;       // T = Ti - d, |d| <= 0.05
     const double* row = tab + i * kBoysN;
     // F_L(T) = sum_k F_{L+k}(Ti) d^k / k!
     double f = row[L + 7];
 #pragma unroll
-    for (int k = 6; k >= 0; --k) f = row[L + k] + f * d / (double)(k + 1);
+    for (int k = 6; k >= 0; --k) f = row[L + k] + f * d * kInvInt[k + 1];
     F[L] = f;
-    const double ex = exp(-T);
+    // e^-T = e^-Ti e^d: the table's last column holds e^-Ti, e^d is its Taylor series to d^9
+    // (remainder below 3e-20)
+    double ed = 1.0;
 #pragma unroll
-    for (int n = L; n >= 1; --n) F[n - 1] = (2.0 * T * F[n] + ex) / (double)(2 * n - 1);
+    for (int k = 9; k >= 1; --k) ed = 1.0 + d * ed * kInvInt[k];
+    const double ex = row[kBoysN - 1] * ed;
+#pragma unroll
+    for (int n = L; n >= 1; --n) F[n - 1] = (2.0 * T * F[n] + ex) * kInvInt[2 * n - 1];
   } else {
-    F[0] = 0.5 * sqrt(M_PI / T);
+    const double inv = 1.0 / T;
+    F[0] = 0.5 * sqrt(M_PI * inv);
 #pragma unroll
-    for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * (double)(2 * n - 1) / (2.0 * T);
+    for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * ((double)(2 * n - 1) * 0.5 * inv);
   }
 }
 
@@ -144,13 +154,15 @@ struct DistLists {
 };
 
 // partial[chunk][pt] = sum over this chunk's slice of every L list
-__global__ __launch_bounds__(kThreads) void k_gto(DistLists lists, const double* __restrict__ boys_tab,
-                                                  const double* __restrict__ pts, GeomView g, int64_t npts,
-                                                  int nchunk, double* __restrict__ partial) {
+// 512 threads share one 42 KB LDS copy of the table (two workgroups per CU; 4 waves per SIMD)
+constexpr int kGtoThreads = 512;
+__global__ __launch_bounds__(kGtoThreads) void k_gto(DistLists lists, const double* __restrict__ boys_tab,
+                                                     const double* __restrict__ pts, GeomView g, int64_t npts,
+                                                     int nchunk, double* __restrict__ partial) {
   __shared__ double tab[kBoysPts * kBoysN];
-  for (int i = threadIdx.x; i < kBoysPts * kBoysN; i += kThreads) tab[i] = boys_tab[i];
+  for (int i = threadIdx.x; i < kBoysPts * kBoysN; i += kGtoThreads) tab[i] = boys_tab[i];
   __syncthreads();
-  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * kGtoThreads + threadIdx.x;
   const int chunk = blockIdx.y;
   double cx, cy, cz;
   const int64_t tt = t < npts ? t : npts - 1;
@@ -215,7 +227,7 @@ std::vector<double> make_boys_table() {
   std::vector<double> tab(kBoysPts * kBoysN);
   for (int i = 0; i < kBoysPts; ++i) {
     const double T = i * kBoysStep;
-    for (int n = 0; n < kBoysN; ++n) {
+    for (int n = 0; n < kBoysN - 1; ++n) {
       // e^-T sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)): positive terms, no cancellation
       long double term = 1.0L / (2 * n + 1), s = term;
       for (int k = 1; k < 2000; ++k) {
@@ -225,6 +237,7 @@ std::vector<double> make_boys_table() {
       }
       tab[i * kBoysN + n] = (double)(std::exp(-(long double)T) * s);
     }
+    tab[i * kBoysN + kBoysN - 1] = (double)std::exp(-(long double)T);
   }
   return tab;
 }
@@ -431,16 +444,16 @@ int gto_eval(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const double* pts, const G
     dl.n[L] = c->n[L];
     total += c->n[L];
   }
-  const int64_t pblocks = (npts + kThreads - 1) / kThreads;
+  const int64_t pblocks = (npts + kGtoThreads - 1) / kGtoThreads;
   // enough workgroups for 256 CUs: split the distributions over chunks when points are few
-  int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(256, 4096 / std::max<int64_t>(pblocks, 1)));
+  int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(256, 2048 / std::max<int64_t>(pblocks, 1)));
   nchunk = std::max(1, std::min(nchunk, std::max(1, total / 64)));
   double* partial = nullptr;
   if ((rc = qm_ws(ctx, WS_GTO, sizeof(double) * (size_t)nchunk * npts, (void**)&partial))) return rc;
   QM_REQUIRE(ctx, pblocks <= 0x7fffffff, QM_ELIMIT, "gto: too many points");
   {
     ProfScope ps(ctx, s, QM_K_GTO);
-    hipLaunchKernelGGL(k_gto, dim3((unsigned)pblocks, nchunk), dim3(kThreads), 0, s, dl, (const double*)c->boys,
+    hipLaunchKernelGGL(k_gto, dim3((unsigned)pblocks, nchunk), dim3(kGtoThreads), 0, s, dl, (const double*)c->boys,
                      pts, gv, npts, nchunk, partial);
   }
   QM_CHECK_LAUNCH(ctx);

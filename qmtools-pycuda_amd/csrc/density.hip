@@ -59,12 +59,37 @@ __device__ __forceinline__ int shell_values(int L, R x, R y, R z, R radial, R* v
     v[2] = (R)C1 * y * radial;
     return 3;
   }
-  v[0] = (R)C20 * ((R)2 * z * z - x * x - y * y) * radial;
+  // 2z^2 - x^2 - y^2 and x^2 - y^2 as products of sums and differences: no cancellation
+  // where they pass through zero (in FP32 the plain forms lose up to ulp(r^2) there)
+  v[0] = (R)C20 * ((z - x) * (z + x) + (z - y) * (z + y)) * radial;
   v[1] = (R)C2A * x * z * radial;
   v[2] = (R)C2A * z * y * radial;
-  v[3] = (R)C22 * (x * x - y * y) * radial;
+  v[3] = (R)C22 * ((x - y) * (x + y)) * radial;
   v[4] = (R)C2A * x * y * radial;
   return 5;
+}
+
+// Calls emit(c, value) for the 2L+1 values of a shell with constant indices (L is
+// uniform over the wave), so no value is picked out of a register array at run time.
+template <typename R, typename Emit>
+__device__ __forceinline__ void shell_emit(int L, R x, R y, R z, R radial, Emit&& emit) {
+  R v[5];
+  if (L == 0) {
+    shell_values<R>(0, x, y, z, radial, v);
+    emit(0, v[0]);
+  } else if (L == 1) {
+    shell_values<R>(1, x, y, z, radial, v);
+    emit(0, v[0]);
+    emit(1, v[1]);
+    emit(2, v[2]);
+  } else {
+    shell_values<R>(2, x, y, z, radial, v);
+    emit(0, v[0]);
+    emit(1, v[1]);
+    emit(2, v[2]);
+    emit(3, v[3]);
+    emit(4, v[4]);
+  }
 }
 
 // Position of chunk point t: sub-point s = t / nvox of voxel v0 + t % nvox (cell centred).
@@ -104,11 +129,19 @@ __device__ __forceinline__ void phi_f64_body(int blk, const MolView& m, const Ge
   } else {
     chunk_point(g, v0, nvox, nsub, t, x, y, z);
   }
+  // shells come grouped by atom: the offset to the nucleus is formed once per atom (uniform test)
+  int atom = -1;
+  double dx = 0, dy = 0, dz = 0, r2 = 0;
   for (int s = 0; s < m.nshell; ++s) {
     const DevShell sh = m.shells[s];
-    const double4 a = m.atoms[sh.atom];
-    const double dx = x - a.x, dy = y - a.y, dz = z - a.z;
-    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (sh.atom != atom) {
+      atom = sh.atom;
+      const double4 a = m.atoms[atom];
+      dx = x - a.x;
+      dy = y - a.y;
+      dz = z - a.z;
+      r2 = dx * dx + dy * dy + dz * dz;
+    }
     double radial = 0.0;
     if (r2 <= (double)sh.rcut2) {
       for (int p = 0; p < sh.nprim; ++p) {
@@ -117,9 +150,8 @@ __device__ __forceinline__ void phi_f64_body(int blk, const MolView& m, const Ge
         if (e < kExpCut) radial += pc.y * exp(-e);   // beyond: below 2e-22 of the coefficient
       }
     }
-    double v[5];
-    const int nc = shell_values<double>(sh.L, dx, dy, dz, radial, v);
-    for (int c = 0; c < nc; ++c) phi[(int64_t)(sh.bf + c) * ld + t] = v[c];
+    double* __restrict__ col = phi + (int64_t)sh.bf * ld + t;
+    shell_emit<double>(sh.L, dx, dy, dz, radial, [&](int c, double val) { col[(int64_t)c * ld] = val; });
   }
   for (int f = m.nbf; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.0;
 }
@@ -158,11 +190,19 @@ __device__ __forceinline__ void phi_f32_body(int blk, const MolView& m, const fl
   // offsets to the nucleus are formed in FP64 and rounded once: rounding x and R
   // separately to FP32 costs ~1e-6 bohr, i.e. ~1e-4 relative error in tight primitives
   float vmax = 0.f;
+  int atom = -1;   // shells come grouped by atom: offsets once per atom
+  float dx = 0.f, dy = 0.f, dz = 0.f, r2 = 0.f;
   for (int s = 0; s < m.nshell; ++s) {
     const DevShell sh = m.shells[s];
-    const double4 a = m.atoms[sh.atom];
-    const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
-    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (sh.atom != atom) {   // r^2 in FP64, rounded once: exp(-a r^2) amplifies its error by a r^2
+      atom = sh.atom;
+      const double4 a = m.atoms[atom];
+      const double ex = x - a.x, ey = y - a.y, ez = z - a.z;
+      dx = (float)ex;
+      dy = (float)ey;
+      dz = (float)ez;
+      r2 = (float)(ex * ex + ey * ey + ez * ez);
+    }
     float radial = 0.f;
     if (r2 <= sh.rcut2) {
       for (int p = 0; p < sh.nprim; ++p) {
@@ -170,12 +210,11 @@ __device__ __forceinline__ void phi_f32_body(int blk, const MolView& m, const fl
         radial += pc.y * __builtin_amdgcn_exp2f(-pc.x * r2);
       }
     }
-    float v[5];
-    const int nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
-    for (int c = 0; c < nc; ++c) {
-      phi[(int64_t)(sh.bf + c) * ld + t] = v[c];
-      vmax = fmaxf(vmax, fabsf(v[c]));
-    }
+    float* __restrict__ col = phi + (int64_t)sh.bf * ld + t;
+    shell_emit<float>(sh.L, dx, dy, dz, radial, [&](int c, float val) {
+      col[(int64_t)c * ld] = val;
+      vmax = fmaxf(vmax, fabsf(val));
+    });
   }
   for (int f = m.nbf; f < nbf_pad; ++f) phi[(int64_t)f * ld + t] = 0.f;
   int e = 0;
@@ -217,8 +256,7 @@ __device__ __forceinline__ void phi_split_body(int blk, const MolView& m, const 
   const float sc = t < npts ? S : 0.f;   // padding points store zeros through the same (uniform) loop
   uint32_t* __restrict__ mine = &stage[0][threadIdx.x];
   int f = 0;   // same sequence in every lane
-  auto push = [&](float val) __attribute__((always_inline)) {
-    const float xs = val * sc;
+  auto push = [&](float xs) __attribute__((always_inline)) {   // xs: the value already times sc
     const T16 hq = (T16)xs;
     const T16 lq = (T16)(xs - (float)hq);
     mine[(f & 7) * kPhiThreads] = bits16(hq) | (bits16(lq) << 16);
@@ -242,11 +280,19 @@ __device__ __forceinline__ void phi_split_body(int blk, const MolView& m, const 
     }
     ++f;
   };
+  int atom = -1;   // shells come grouped by atom: offsets once per atom
+  float dx = 0.f, dy = 0.f, dz = 0.f, r2 = 0.f;
   for (int s = 0; s < m.nshell; ++s) {
     const DevShell sh = m.shells[s];
-    const double4 a = m.atoms[sh.atom];
-    const float dx = (float)(x - a.x), dy = (float)(y - a.y), dz = (float)(z - a.z);
-    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (sh.atom != atom) {   // r^2 in FP64, rounded once: exp(-a r^2) amplifies its error by a r^2
+      atom = sh.atom;
+      const double4 a = m.atoms[atom];
+      const double ex = x - a.x, ey = y - a.y, ez = z - a.z;
+      dx = (float)ex;
+      dy = (float)ey;
+      dz = (float)ez;
+      r2 = (float)(ex * ex + ey * ey + ez * ez);
+    }
     float radial = 0.f;
     if (r2 <= sh.rcut2) {
       for (int p = 0; p < sh.nprim; ++p) {
@@ -254,9 +300,8 @@ __device__ __forceinline__ void phi_split_body(int blk, const MolView& m, const 
         radial += pc.y * __builtin_amdgcn_exp2f(-pc.x * r2);
       }
     }
-    float v[5];
-    const int nc = shell_values<float>(sh.L, dx, dy, dz, radial, v);
-    for (int c = 0; c < nc; ++c) push(v[c]);
+    // S is a power of two: scaling the radial part scales every value exactly
+    shell_emit<float>(sh.L, dx, dy, dz, radial * sc, [&](int, float val) { push(val); });
   }
   while (f < nbf_pad) push(0.f);
 }

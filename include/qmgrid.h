@@ -245,10 +245,14 @@ int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* geoms, cons
 
 /* Tile-shape knobs of the density contraction and the Jacobi sweep (no reference counterpart). */
 enum {
-  QM_TUNE_SPLIT_VARIANT = 0, /* 16-bit split contraction: 0 = 128 x 128 tiles with per-block partial sums,
-                               64 or 128 = points per workgroup of the all-columns kernel */
+  QM_TUNE_SPLIT_VARIANT = 0, /* 16-bit split contraction, 128 x 128 tiles with per-block partial sums:
+                               0 / 2 = FP32 basis values split while staged (8 / 4 waves), 1 = basis values
+                               written pre-split, staged through registers, 3 = pre-split, fetched by LDS DMA
+                               (default); 64 or 128 = points per workgroup of the all-columns kernel */
   QM_TUNE_F64_VARIANT = 1,  /* FP64 contraction: 0 = all columns per workgroup (wave-cyclic tiles), 1 = per-wave
-                               loads of the dense P, no barriers, 2 = 128 x 128 tiles with per-block partial sums */
+                               loads of the dense P, no barriers, 2 = 128 x 128 tiles with per-block partial sums,
+                               3 = 128 x 64 tiles with diagonal skipping, 4 = 128 x 128 tiles fed by LDS DMA with
+                               diagonal skipping (default) */
   QM_TUNE_JACOBI_ZS = 2,    /* z planes per Jacobi sweep workgroup, 0 = chosen per grid, else 1..4096 */
   QM_TUNE_COUNT = 3
 };

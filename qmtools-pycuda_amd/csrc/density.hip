@@ -643,8 +643,8 @@ __device__ __forceinline__ void f64_rowdot(const d4 (&acc)[2][4], const double* 
     for (int c = 0; c < 4; ++c) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int n = c0 + wc * 64 + c * 16 + lk + 4 * q;
-        if (n < np) p += acc[r][c][q] * phi[(int64_t)n * ld + row0 + tl];
+        const int n = c0 + wc * 64 + c * 16 + lk + 4 * q;   // c0 < 0 only for right-aligned blocks
+        if (n >= 0 && n < np) p += acc[r][c][q] * phi[(int64_t)n * ld + row0 + tl];
       }
     }
     p += __shfl_xor(p, 16);
@@ -728,21 +728,26 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
 // right after the barrier that opens the previous stage.  The loads come from
 // inline assembly (lds_dma16), the wait before each barrier is explicit.
 __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double* __restrict__ phi, int ld,
-                                                   const double* __restrict__ M, int np, double* __restrict__ part,
-                                                   int ldp) {
+                                                   const double* __restrict__ M, int np, int nbf, int ncb,
+                                                   double* __restrict__ part, int ldp) {
   __shared__ __align__(16) double ring[2][2][kTKS][kTT + kTPad];
   __shared__ double red[2][kTT];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   // waves w and w + 4 (one SIMD when waves are dealt round-robin) take the two column halves,
-  // so the MFMAs skipped in the diagonal block below are spread evenly over the SIMDs
+  // so the MFMAs skipped below are spread evenly over the SIMDs
   const int wr = wave & 3, wc = wave >> 2;
   const int li = lane & 15, lk = lane >> 4;
-  const int row0 = rt * kTT, c0 = cb * kTT;
-  const int nst = min(np, c0 + kTT) / kTKS;
+  // Column blocks are right-aligned on ne = nbf rounded up to 16: block cb covers columns
+  // c0 .. c0 + 127 with c0 = 128 cb - shift, so only block 0 (the cheapest, k < c0 + 128) holds
+  // padding, as columns below 0; rows and columns of M from nbf on are zero and are not visited.
+  const int ne = (nbf + 15) / 16 * 16;
+  const int row0 = rt * kTT, c0 = cb * kTT - (ncb * kTT - ne);
+  const int nst = (c0 + kTT) / kTKS;
+  const int cpad = min(max(-(c0 + 64 * wc) / 16, 0), 4);   // tile columns of this wave below column 0
   // wave w brings k rows 4 (w % 4) .. 4 (w % 4) + 3 of operand w / 4 (0: basis values, 1: M)
   const int op = wave >> 2, k0 = 4 * (wave & 3);
-  const double* src = op == 0 ? phi + row0 + 2 * lane : M + min(c0 + 2 * lane, np - 2);   // past np: never used
+  const double* src = op == 0 ? phi + row0 + 2 * lane : M + min(max(c0 + 2 * lane, 0), np - 2);   // clamped: unused
   const int64_t lds_ld = op == 0 ? ld : np;
   auto issue = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
@@ -762,10 +767,11 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
     __syncthreads();                                    // everyone's loads of st; everyone done with st - 1
     if (st + 1 < nst) issue(st + 1);
     const int buf = st & 1;
-    // M is zero below its diagonal: in the block's diagonal stages (j = st - c0 / 16 >= 0) tile
-    // column c (columns c0 + 64 wc + 16 c ..) needs this stage only while 16 j <= 64 wc + 16 c,
-    // i.e. c >= j - 4 wc.  The first needed column is uniform per wave; each case is its own
-    // straight-line body so the accumulators stay in place.
+    // M is zero below its diagonal: in the block's diagonal stages tile column c (columns
+    // c0 + 64 wc + 16 c ..) needs stage st only while 16 st <= c0 + 64 wc + 16 c, i.e.
+    // c >= st - c0 / 16 - 4 wc; tile columns below column 0 are never needed.  The first needed
+    // tile column is uniform per wave; each case is its own straight-line body so the
+    // accumulators stay in place.
     auto body = [&](auto first) __attribute__((always_inline)) {
       constexpr int cs = decltype(first)::value;
 #pragma unroll
@@ -782,8 +788,9 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
             acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c], a[r], acc[r][c], 0, 0, 0);
       }
     };
-    const int cs = min(max(st - c0 / kTKS - 4 * wc, 0), 4);
-    // (a skipped tile would only add products with M's zeros, so cases may round cs down)
+    const int cs = max(min(max(st - c0 / kTKS - 4 * wc, 0), 4), cpad);
+    // (a tile computed anyway only adds products with M's zeros, or, below column 0, results
+    // the row-dot ignores, so cs is rounded down)
     if (cs < 2) body(std::integral_constant<int, 0>());
     else if (cs < 4) body(std::integral_constant<int, 2>());
   }
@@ -1189,11 +1196,11 @@ __global__ __launch_bounds__(kTWG) void k_contract_f64_rows(const double* __rest
 
 template <bool DMA>
 __global__ __launch_bounds__(kTWG) void k_contract_f64_tiles(const double* __restrict__ phi, int ld,
-                                                             const double* __restrict__ M, int np, int nrt, int ncb,
-                                                             double* __restrict__ part) {
+                                                             const double* __restrict__ M, int np, int nbf, int nrt,
+                                                             int ncb, double* __restrict__ part) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  if (DMA) tiled_f64_dma_body(rt, cb, phi, ld, M, np, part, ld);
+  if (DMA) tiled_f64_dma_body(rt, cb, phi, ld, M, np, nbf, ncb, part, ld);
   else tiled_f64_body(rt, cb, phi, ld, M, np, part, ld);
 }
 
@@ -1522,10 +1529,10 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
                            (const double*)mol->M64, np, nrt, ncb, part);
       else if (ctx->f64_variant == 4)
         hipLaunchKernelGGL(k_contract_f64_tiles<true>, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s,
-                           (const double*)phi, ld, (const double*)mol->M64, np, nrt, ncb, part);
+                           (const double*)phi, ld, (const double*)mol->M64, np, mol->nbf, nrt, ncb, part);
       else
         hipLaunchKernelGGL(k_contract_f64_tiles<false>, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s,
-                           (const double*)phi, ld, (const double*)mol->M64, np, nrt, ncb, part);
+                           (const double*)phi, ld, (const double*)mol->M64, np, mol->nbf, nrt, ncb, part);
       QM_CHECK_LAUNCH(ctx);
       if (nsub3 == 1) {
         hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncb,
@@ -1989,7 +1996,9 @@ __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __re
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
-  if (DMA) tiled_f64_dma_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
+  if (DMA)
+    tiled_f64_dma_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.mol.nbf, q.ncb, q.part,
+                       q.ld);
   else tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 

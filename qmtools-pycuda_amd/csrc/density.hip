@@ -825,7 +825,7 @@ __device__ __forceinline__ void presplit_rowdot(const f4 (&acc)[RT][CT], const T
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
       const int n0 = c0 + wc * TCW + c * 16 + 4 * lk;   // a multiple of 4: n0 .. n0+3 share one 8-group
-      if (n0 < np) {
+      if (n0 >= 0 && n0 < np) {   // c0 < 0 only for right-aligned blocks
         // element (n, t) of a plane sits at ((n / 8) ld + t) 8 + n % 8
         const int64_t o = ((int64_t)(n0 >> 3) * ld + row0 + tl) * 8 + (n0 & 7);
         const V4 h = __builtin_bit_cast(V4, *(const uint2*)(a_hi + o));
@@ -1015,7 +1015,7 @@ __global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const voi
 template <typename T16>
 __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __restrict__ a_hi, int ld,
                                                const T16* __restrict__ p_hi, const T16* __restrict__ p_lo, int np,
-                                               double* __restrict__ part, int ldp) {
+                                               int nbf, int ncb, double* __restrict__ part, int ldp) {
   typedef typename Vec8<T16>::type V8;
   constexpr int WR = 4, WC = 2, TRW = kTT / WR, TCW = kTT / WC, RT = TRW / 16, CT = TCW / 16;
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -1027,11 +1027,15 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wr = wave & 3, wc = wave >> 2;   // waves w, w + 4 (one SIMD) take the two column halves
   const int li = lane & 15, lk = lane >> 4;
-  const int row0 = rt * kTT, c0 = cb * kTT;
-  const int nst = min(np, c0 + kTT) / 32;
+  // column blocks right-aligned on ne = nbf rounded up to 32 (as in tiled_f64_dma_body): only
+  // block 0 holds padding, as columns below 0, skipped like tiles below the diagonal
+  const int ne = (nbf + 31) / 32 * 32;
+  const int row0 = rt * kTT, c0 = cb * kTT - (ncb * kTT - ne);
+  const int nst = (c0 + kTT) / 32;
+  const int cpad = min(max(-(c0 + 64 * wc) / 16, 0), 4);   // tile columns of this wave below column 0
   // thread t fetches word t of each plane: k group g = t / 128, row / column ii = t % 128
   const int g = t / kTT, ii = t % kTT;
-  const int bcol = min(c0 + ii, np - 1);   // columns past the matrix are never used
+  const int bcol = min(max(c0 + ii, 0), np - 1);   // clamped columns are never used
   auto issue = [&](int st) __attribute__((always_inline)) {
     const int slot = st & 1;
     const int64_t oa = ((int64_t)(st * 4 + g) * ld + row0 + ii) * 8;
@@ -1077,7 +1081,7 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
           acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
         }
     };
-    const int cs = min(max(2 * (st - c0 / 32) - 4 * wc, 0), 4);
+    const int cs = max(min(max(2 * (st - c0 / 32) - 4 * wc, 0), 4), cpad);
     if (cs < 2) body(std::integral_constant<int, 0>());
     else if (cs < 4) body(std::integral_constant<int, 2>());
   }
@@ -1089,10 +1093,11 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
 template <typename T16>
 __global__ __launch_bounds__(512) void k_contract_split_dma(const void* __restrict__ phi, int ld,
                                                             const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
-                                                            int np, int nrt, int ncb, double* __restrict__ part) {
+                                                            int np, int nbf, int nrt, int ncb,
+                                                            double* __restrict__ part) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  tiled_dma_body<T16>(rt, cb, (const T16*)phi, ld, p_hi, p_lo, np, part, ld);
+  tiled_dma_body<T16>(rt, cb, (const T16*)phi, ld, p_hi, p_lo, np, nbf, ncb, part, ld);
 }
 
 // Variant 3: tiles of 128 points x 64 columns with waves stacked 8 x 1 (wave w
@@ -1588,7 +1593,7 @@ inline bool split_pre(int v) { return v == 1 || v == 3; }
 
 template <typename T16>
 void launch_split_tiles(hipStream_t s, int v, int blocks, const void* phi, int ld, const float* pscale,
-                        const T16* Ph, const T16* Pl, int np, int nrt, int ncb, double* part) {
+                        const T16* Ph, const T16* Pl, int np, int nbf, int nrt, int ncb, double* part) {
   switch (v) {
     case 1:
       hipLaunchKernelGGL((k_contract_split_tiles<T16, 4, 2, true>), dim3(blocks), dim3(512), 0, s, phi, ld, pscale,
@@ -1599,7 +1604,7 @@ void launch_split_tiles(hipStream_t s, int v, int blocks, const void* phi, int l
                          np, nrt, ncb, part);
       break;
     case 3:
-      hipLaunchKernelGGL(k_contract_split_dma<T16>, dim3(blocks), dim3(512), 0, s, phi, ld, Ph, Pl, np, nrt, ncb,
+      hipLaunchKernelGGL(k_contract_split_dma<T16>, dim3(blocks), dim3(512), 0, s, phi, ld, Ph, Pl, np, nbf, nrt, ncb,
                          part);
       break;
     default:
@@ -1661,7 +1666,8 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
     if (tiled) {
       const int nrt = ld / kTT;
-      launch_split_tiles<T16>(s, ctx->split_tm, tile_grid(nrt, ncb), phi, ld, pscale, Phi, Plo, np, nrt, ncb, part);
+      launch_split_tiles<T16>(s, ctx->split_tm, tile_grid(nrt, ncb), phi, ld, pscale, Phi, Plo, np, mol->nbf, nrt, ncb,
+                              part);
       QM_CHECK_LAUNCH(ctx);
       // pre-split: both factors of the row-dot carry S, divided out here with the volume element
       const float* ps = pre ? nullptr : (const float*)pscale;
@@ -2016,7 +2022,8 @@ __global__ __launch_bounds__(512) void kb_contract_split_dma(const BChunk* __res
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
-  tiled_dma_body<T16>(rt, cb, (const T16*)q.phi, q.ld, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad, q.part, q.ld);
+  tiled_dma_body<T16>(rt, cb, (const T16*)q.phi, q.ld, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad, q.mol.nbf, q.ncb,
+                      q.part, q.ld);
 }
 
 template <typename T16>

@@ -43,6 +43,26 @@ __device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
   return Cx<R>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
+// LDS index skew: one empty slot after every W complex values, W = 256 bytes / value size
+// (16 for complex128, 32 for complex64), so that a W-value run spans the 64 banks once and
+// the strided accesses of the Stockham stages (stride RR in the first stage's writes) and of
+// the twiddle table land on distinct banks.  Lines are LS slots apart, LS = W/8 mod W, so the
+// 8 lines that one LDS cycle serves in the load and store phases of a strided axis (8 lines
+// x W/8 consecutive points) do not share banks either.
+template <typename R>
+struct Skew {
+  static constexpr int S = sizeof(R) == 8 ? 4 : 5;   // log2 W
+};
+template <typename R>
+__host__ __device__ __forceinline__ int sk(int i) { return i + (i >> Skew<R>::S); }
+template <typename R>
+__host__ __device__ __forceinline__ int line_slots(int n) {
+  constexpr int W = 1 << Skew<R>::S;
+  return (n + (n >> Skew<R>::S) + W - 1) / W * W + W / 8;
+}
+template <typename R>
+__host__ __device__ __forceinline__ int table_slots(int n) { return n + (n >> Skew<R>::S) + 1; }
+
 struct AxisPlan {
   int n;              // transform length
   int nstage;
@@ -68,7 +88,7 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
   } else if (RR == 4) {
     const Cx<R> a0{x[0].x + x[2].x, x[0].y + x[2].y}, a1{x[0].x - x[2].x, x[0].y - x[2].y};
     const Cx<R> b0{x[1].x + x[3].x, x[1].y + x[3].y}, b1{x[1].x - x[3].x, x[1].y - x[3].y};
-    const Cx<R> b1w = cmul(b1, tw[w1]);   // exp(-+ i pi/2)
+    const Cx<R> b1w = cmul(b1, tw[sk<R>(w1)]);   // exp(-+ i pi/2)
     x[0] = Cx<R>{a0.x + b0.x, a0.y + b0.y};
     x[1] = Cx<R>{a1.x + b1w.x, a1.y + b1w.y};
     x[2] = Cx<R>{a0.x - b0.x, a0.y - b0.y};
@@ -81,10 +101,10 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
       a[q] = Cx<R>{x[q].x + x[q + 4].x, x[q].y + x[q + 4].y};
       b[q] = Cx<R>{x[q].x - x[q + 4].x, x[q].y - x[q + 4].y};
     }
-    b[1] = cmul(b[1], tw[w1]);
-    b[2] = cmul(b[2], tw[2 * w1]);
-    b[3] = cmul(b[3], tw[3 * w1]);
-    const Cx<R> w4 = tw[2 * w1];
+    b[1] = cmul(b[1], tw[sk<R>(w1)]);
+    b[2] = cmul(b[2], tw[sk<R>(2 * w1)]);
+    b[3] = cmul(b[3], tw[sk<R>(3 * w1)]);
+    const Cx<R> w4 = tw[sk<R>(2 * w1)];
     auto dft4 = [&](const Cx<R>* c, Cx<R>& o0, Cx<R>& o1, Cx<R>& o2, Cx<R>& o3) {
       const Cx<R> d0{c[0].x + c[2].x, c[0].y + c[2].y}, d1{c[0].x - c[2].x, c[0].y - c[2].y};
       const Cx<R> d2{c[1].x + c[3].x, c[1].y + c[3].y};
@@ -103,7 +123,7 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
       Cx<R> acc = x[0];
 #pragma unroll
       for (int q = 1; q < RR; ++q) {
-        const Cx<R> t = cmul(x[q], tw[((j * q) % RR) * w1]);
+        const Cx<R> t = cmul(x[q], tw[sk<R>(((j * q) % RR) * w1)]);
         acc.x += t.x;
         acc.y += t.y;
       }
@@ -118,6 +138,7 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
 // are read into registers, then (after a barrier) the outputs are written.
 template <typename R, int RR>
 __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restrict__ tw, int n, int B, int p) {
+  const int LS = line_slots<R>(n);
   constexpr int MAXB = (4096 / kFftThreads + RR - 1) / RR;   // B * n <= 4096: at most ceil(8/RR) butterflies per thread
   const int m = n / RR;
   const int tstep = n / (p * RR);
@@ -128,11 +149,11 @@ __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restric
     const int bf = threadIdx.x + b * kFftThreads;
     if (bf < total) {
       const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
-      const Cx<R>* src = buf + l * n;
+      const Cx<R>* src = buf + l * LS;
 #pragma unroll
       for (int j = 0; j < RR; ++j) {
-        Cx<R> v = src[i + j * m];
-        if (k) v = cmul(v, tw[(j * k * tstep) % n]);
+        Cx<R> v = src[sk<R>(i + j * m)];
+        if (k) v = cmul(v, tw[sk<R>((j * k * tstep) % n)]);
         x[b][j] = v;
       }
       dft<R, RR>(x[b], tw, m);
@@ -144,9 +165,10 @@ __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restric
     const int bf = threadIdx.x + b * kFftThreads;
     if (bf < total) {
       const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
-      Cx<R>* dst = buf + l * n + (i - k) * RR + k;
+      Cx<R>* dst = buf + l * LS;
+      const int o = (i - k) * RR + k;
 #pragma unroll
-      for (int j = 0; j < RR; ++j) dst[j * p] = x[b][j];
+      for (int j = 0; j < RR; ++j) dst[sk<R>(o + j * p)] = x[b][j];
     }
   }
   __syncthreads();
@@ -158,20 +180,22 @@ __device__ __forceinline__ void stage_generic(const Cx<R>* in, Cx<R>* out, const
                                               int B, int RR, int p) {
   const int m = n / RR;
   const int tstep = n / (p * RR);
+  const int LS = line_slots<R>(n);
   for (int bf = threadIdx.x; bf < B * m; bf += kFftThreads) {
     const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
-    const Cx<R>* src = in + l * n;
-    Cx<R>* dst = out + l * n + (i - k) * RR + k;
+    const Cx<R>* src = in + l * LS;
+    Cx<R>* dst = out + l * LS;
+    const int o = (i - k) * RR + k;
     for (int j = 0; j < RR; ++j) {
       Cx<R> acc{0, 0};
       for (int q = 0; q < RR; ++q) {
-        Cx<R> v = src[i + q * m];
-        if (k) v = cmul(v, tw[(q * k * tstep) % n]);
-        const Cx<R> t = cmul(v, tw[((j * q) % RR) * m]);
+        Cx<R> v = src[sk<R>(i + q * m)];
+        if (k) v = cmul(v, tw[sk<R>((q * k * tstep) % n)]);
+        const Cx<R> t = cmul(v, tw[sk<R>(((j * q) % RR) * m)]);
         acc.x += t.x;
         acc.y += t.y;
       }
-      dst[j * p] = acc;
+      dst[sk<R>(o + j * p)] = acc;
     }
   }
   __syncthreads();
@@ -187,9 +211,10 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
   extern __shared__ unsigned char smem[];
   const int n = plan.n;
   const int B = ln.B;
+  const int LS = line_slots<R>(n);
   Cx<R>* tw = (Cx<R>*)smem;
-  Cx<R>* buf0 = tw + n;
-  Cx<R>* buf1 = buf0 + (int64_t)B * n;
+  Cx<R>* buf0 = tw + table_slots<R>(n);
+  Cx<R>* buf1 = buf0 + (int64_t)B * LS;
   const int lines_per_row = (ln.nl1 + B - 1) / B;
   const int64_t wg = blockIdx.x;
   const int64_t l0 = wg / lines_per_row;
@@ -197,7 +222,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
   for (int q = threadIdx.x; q < n; q += kFftThreads) {
     Cx<R> w = twg[q];
     if (INV) w.y = -w.y;
-    tw[q] = w;
+    tw[sk<R>(q)] = w;
   }
   const int64_t base0 = l0 * ln.s0 + (int64_t)l1b * ln.s1;
   const int nb = min(B, ln.nl1 - l1b);
@@ -232,7 +257,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
         l = e % B;
         a = e / B;
       }
-      buf0[l * n + a] = ld[q];
+      buf0[l * LS + sk<R>(a)] = ld[q];
     }
   }
   __syncthreads();
@@ -278,7 +303,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
       l = e % B;
       a = e / B;
     }
-    if (l < nb) data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride] = res[l * n + a];
+    if (l < nb) data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride] = res[l * LS + sk<R>(a)];
   }
 }
 
@@ -484,7 +509,9 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
   ln.B = B;
   const int lines_per_row = (ln.nl1 + B - 1) / B;
   const int64_t nwg = (int64_t)ln.nl0 * lines_per_row;
-  const size_t shmem = sizeof(Cx<R>) * (size_t)n * (1 + (size_t)planes * B);
+  const size_t shmem = sizeof(Cx<R>) * ((size_t)table_slots<R>(n) + (size_t)planes * B * line_slots<R>(n));
+  QM_REQUIRE(ctx, shmem <= (size_t)128 * 1024, QM_ELIMIT, "fft: length %d needs %zu bytes of LDS per workgroup", n,
+             shmem);
   QM_REQUIRE(ctx, nwg <= 0x7fffffff, QM_ELIMIT, "fft: too many lines");
   if (nwg == 0) return QM_OK;
   const bool pow2 = (n & (n - 1)) == 0;

@@ -136,24 +136,39 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
 
 // One radix-RR Stockham stage in place: all inputs of the thread's butterflies
 // are read into registers, then (after a barrier) the outputs are written.
-template <typename R, int RR>
+// POW2: n, p and RR are powers of two, so the line / butterfly / twiddle indices come from
+// shifts and masks instead of integer divisions by run-time values.
+template <typename R, int RR, bool POW2 = false>
 __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restrict__ tw, int n, int B, int p) {
   const int LS = line_slots<R>(n);
   constexpr int MAXB = (4096 / kFftThreads + RR - 1) / RR;   // B * n <= 4096: at most ceil(8/RR) butterflies per thread
   const int m = n / RR;
   const int tstep = n / (p * RR);
   const int total = B * m;
+  const int lm = POW2 ? __builtin_ctz(m) : 0;
+  auto split = [&](int bf, int& l, int& i, int& k) __attribute__((always_inline)) {
+    if (POW2) {
+      l = bf >> lm;
+      i = bf & (m - 1);
+      k = i & (p - 1);
+    } else {
+      l = bf / m;
+      i = bf - l * m;
+      k = i % p;
+    }
+  };
   Cx<R> x[MAXB][RR];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int bf = threadIdx.x + b * kFftThreads;
     if (bf < total) {
-      const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
+      int l, i, k;
+      split(bf, l, i, k);
       const Cx<R>* src = buf + l * LS;
 #pragma unroll
       for (int j = 0; j < RR; ++j) {
         Cx<R> v = src[sk<R>(i + j * m)];
-        if (k) v = cmul(v, tw[sk<R>((j * k * tstep) % n)]);
+        if (k) v = cmul(v, tw[sk<R>(POW2 ? (j * k * tstep) & (n - 1) : (j * k * tstep) % n)]);
         x[b][j] = v;
       }
       dft<R, RR>(x[b], tw, m);
@@ -164,7 +179,8 @@ __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restric
   for (int b = 0; b < MAXB; ++b) {
     const int bf = threadIdx.x + b * kFftThreads;
     if (bf < total) {
-      const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
+      int l, i, k;
+      split(bf, l, i, k);
       Cx<R>* dst = buf + l * LS;
       const int o = (i - k) * RR + k;
 #pragma unroll
@@ -226,6 +242,19 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
   }
   const int64_t base0 = l0 * ln.s0 + (int64_t)l1b * ln.s1;
   const int nb = min(B, ln.nl1 - l1b);
+  // element e of the workgroup's B lines: line l, position a (line-major on the contiguous axis,
+  // line-minor otherwise, so a load instruction reads B adjacent lines); POW2 plans have B a
+  // power of two as well, and the split is shifts and masks
+  const int lgn = POW2 ? __builtin_ctz(n) : 0, lgb = POW2 ? __builtin_ctz(B) : 0;
+  auto split = [&](int e, int& l, int& a) __attribute__((always_inline)) {
+    if (ln.stride == 1) {
+      l = POW2 ? e >> lgn : e / n;
+      a = e - l * n;
+    } else {
+      l = POW2 ? e & (B - 1) : e % B;
+      a = POW2 ? e >> lgb : e / B;
+    }
+  };
   // all of a thread's loads are issued before any is used (B n <= 4096: 8 per thread)
   constexpr int kPer = 4096 / kFftThreads;
   Cx<R> ld[kPer];
@@ -235,13 +264,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
     ld[q] = Cx<R>{0, 0};
     if (e < B * n) {
       int l, a;
-      if (ln.stride == 1) {
-        l = e / n;
-        a = e - l * n;
-      } else {
-        l = e % B;
-        a = e / B;
-      }
+      split(e, l, a);
       if (l < nb) ld[q] = data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride];
     }
   }
@@ -250,13 +273,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
     const int e = threadIdx.x + q * kFftThreads;
     if (e < B * n) {
       int l, a;
-      if (ln.stride == 1) {
-        l = e / n;
-        a = e - l * n;
-      } else {
-        l = e % B;
-        a = e / B;
-      }
+      split(e, l, a);
       buf0[l * LS + sk<R>(a)] = ld[q];
     }
   }
@@ -266,9 +283,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
     for (int s = 0; s < plan.nstage; ++s) {
       const int p = plan.span[s];
       switch (plan.radix[s]) {
-        case 2: stage_inplace<R, 2>(buf0, tw, n, B, p); break;
-        case 4: stage_inplace<R, 4>(buf0, tw, n, B, p); break;
-        default: stage_inplace<R, 8>(buf0, tw, n, B, p); break;
+        case 2: stage_inplace<R, 2, true>(buf0, tw, n, B, p); break;
+        case 4: stage_inplace<R, 4, true>(buf0, tw, n, B, p); break;
+        default: stage_inplace<R, 8, true>(buf0, tw, n, B, p); break;
       }
     }
   } else if (inplace) {
@@ -296,13 +313,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
   }
   for (int e = threadIdx.x; e < B * n; e += kFftThreads) {
     int l, a;
-    if (ln.stride == 1) {
-      l = e / n;
-      a = e - l * n;
-    } else {
-      l = e % B;
-      a = e / B;
-    }
+    split(e, l, a);
     if (l < nb) data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride] = res[l * LS + sk<R>(a)];
   }
 }
@@ -506,6 +517,9 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
   int B = std::max(1, std::min(16, 4096 / n));
   if (!inplace) B = std::max(1, std::min(B, 2048 / n));
   B = std::max(1, std::min(B, ln.nl1));
+  const bool pow2 = (n & (n - 1)) == 0;
+  if (pow2)
+    while (B & (B - 1)) B &= B - 1;   // the power-of-two kernel splits indices by shifts
   ln.B = B;
   const int lines_per_row = (ln.nl1 + B - 1) / B;
   const int64_t nwg = (int64_t)ln.nl0 * lines_per_row;
@@ -514,7 +528,6 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
              shmem);
   QM_REQUIRE(ctx, nwg <= 0x7fffffff, QM_ELIMIT, "fft: too many lines");
   if (nwg == 0) return QM_OK;
-  const bool pow2 = (n & (n - 1)) == 0;
   static bool attr_set = false;
   if (!attr_set) {
     QM_CHECK_HIP(ctx, hipFuncSetAttribute((const void*)k_fft_axis<R, INV, false>,

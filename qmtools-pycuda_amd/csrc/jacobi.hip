@@ -57,6 +57,8 @@ __device__ __forceinline__ void strip_body(int bx, int by, int bz, int zs, const
   auto row = [&](int j) -> size_t { return sy * (size_t)min(max(j, 0), ny - 1); };
   auto pln = [&](int k) -> size_t { return sz * (size_t)min(max(k, 0), nz - 1); };
   const bool edge = lane == 0 || lane == kSW - 1;
+  // multiplications by rounded reciprocals instead of IEEE divisions (about 10 instructions each)
+  const T hinv = (T)1 / h;
   T pm[kSR + 2], pc[kSR + 2], pp[kSR + 2], hc[kSR], rr[kSR];
 #pragma unroll
   for (int q = 0; q < kSR + 2; ++q) {
@@ -101,9 +103,9 @@ __device__ __forceinline__ void strip_body(int bx, int by, int bz, int zs, const
       sum += pc[r];
       sum += pp[r + 1];
       sum += pm[r + 1];
-      sum -= rr[r] / h;
+      sum -= rr[r] * hinv;
       const bool outer = zb || i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
-      const T val = outer ? c : sum / (T)6;
+      const T val = outer ? c : sum * (T)(1.0 / 6.0);
       if (i < nx && j < ny) vout[pk + sy * (size_t)j + (unsigned)i] = val;
     }
 #pragma unroll
@@ -160,6 +162,8 @@ __device__ __forceinline__ void strip2_body(int bx, int by, int bz, int zs, cons
   auto row = [&](int j) -> size_t { return sy * (size_t)min(max(j, 0), ny - 1); };
   auto pln = [&](int k) -> size_t { return sz * (size_t)min(max(k, 0), nz - 1); };
   const bool edge = lane == 0 || lane == kSW - 1;
+  // multiplications by rounded reciprocals instead of IEEE divisions (about 10 instructions each)
+  const T hinv = (T)1 / h;
   const bool xb = i <= 0 || i >= nx - 1;            // outer layer (or outside) in x
   T am[R0], ac[R0], ap[R0];         // V on planes m-1, m, m+1, rows j0-2+q
   T bmm[R1], bm[R1];                // first sweep on planes m-2, m-1, rows j0-1+p
@@ -214,9 +218,9 @@ __device__ __forceinline__ void strip2_body(int bx, int by, int bz, int zs, cons
       sum += ac[p];
       sum += ap[p + 1];
       sum += am[p + 1];
-      sum -= rc[p] / h;
+      sum -= rc[p] * hinv;
       const bool outer = zb1 || xb || j <= 0 || j >= ny - 1;
-      bc[p] = outer ? c : sum / (T)6;
+      bc[p] = outer ? c : sum * (T)(1.0 / 6.0);
     }
     // second sweep on plane m-1
     const int k = m - 1;
@@ -233,9 +237,9 @@ __device__ __forceinline__ void strip2_body(int bx, int by, int bz, int zs, cons
         sum += bm[r];
         sum += bc[r + 1];
         sum += bmm[r + 1];
-        sum -= rm[r + 1] / h;
+        sum -= rm[r + 1] * hinv;
         const bool outer = zb2 || xb || j == 0 || j == ny - 1;
-        const T val = outer ? c : sum / (T)6;
+        const T val = outer ? c : sum * (T)(1.0 / 6.0);
         if (lane >= 1 && lane <= kSW - 2 && i < nx && j < ny) vout[sz * (size_t)k + sy * (size_t)j + (unsigned)i] = val;
       }
     }

@@ -182,17 +182,28 @@ __global__ __launch_bounds__(256) void k_vne_sweep(const T* __restrict__ v, T* _
   const double self = 1.0 - dif * kVneSelf;
   const int gx = x0 + tx;
   double acc = 0.0;
+  // the thread's two points (tile rows 2 ty + 1, 2 ty + 2, column tx + 1) need rows 2 ty .. 2 ty + 3
+  // and columns tx .. tx + 2 of planes k-1, k, k+1: kept in registers and shifted along z, so each
+  // plane's 12 values are read from LDS once instead of 27 reads per point
+  T blk[3][4][3];
+  auto read_block = [&](const T* P, T (&b)[4][3]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) b[r][c] = P[(2 * ty + r) * kVW + tx + c];
+  };
+  read_block(pl[(z0 - 1) & 3], blk[0]);
+  read_block(pl[z0 & 3], blk[1]);
   for (int k = z0; k < z1; ++k) {
     T nxt[kVLoads];
     const int64_t pk2 = plane_at(k + 2);
 #pragma unroll
     for (int u = 0; u < kVLoads; ++u) nxt[u] = slot_e[u] < kVPlane ? v[pk2 + off[u]] : (T)0;
-    const T* P[3] = {pl[(k - 1) & 3], pl[k & 3], pl[(k + 1) & 3]};
+    read_block(pl[(k + 1) & 3], blk[2]);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      const int ly = 2 * ty + q + 1, gy = y0 + 2 * ty + q;
-      const int lc = ly * kVW + tx + 1;
-      const double c = (double)P[1][lc];
+      const int gy = y0 + 2 * ty + q;
+      const double c = (double)blk[1][q + 1][1];
       double sum = 0.0;
 #pragma unroll
       for (int x = -1; x <= 1; ++x)
@@ -203,7 +214,7 @@ __global__ __launch_bounds__(256) void k_vne_sweep(const T* __restrict__ v, T* _
             const int d = (x != 0) + (y != 0) + (z != 0);
             if (d == 0) continue;
             const double w = d == 1 ? w1 : (d == 2 ? w2 : w3);
-            sum += w * (double)P[z + 1][lc + y * kVW + x];
+            sum += w * (double)blk[z + 1][q + 1 + y][1 + x];
           }
       sum += self * c;
       sum -= c * ads;
@@ -225,6 +236,13 @@ __global__ __launch_bounds__(256) void k_vne_sweep(const T* __restrict__ v, T* _
 #pragma unroll
     for (int u = 0; u < kVLoads; ++u)
       if (slot_e[u] < kVPlane) pl[(k + 2) & 3][slot_e[u]] = nxt[u];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        blk[0][r][c] = blk[1][r][c];
+        blk[1][r][c] = blk[2][r][c];
+      }
     __syncthreads();
   }
   if (part) {

@@ -30,7 +30,14 @@ constexpr double kRBohr = 0.529177211;       // kernel_fft.cu:15
 constexpr double kEps0 = 0.0055263494;       // kernel_fft.cu:16, e/(V Angstrom)
 constexpr double kAng2Bor = 1.8897259886;
 constexpr int kThreads = 256;
-constexpr int kFftThreads = 512;   // threads of an axis-pass workgroup (B n <= 4096: 8 points per thread)
+constexpr int kFftThreads = 512;   // threads of an axis-pass workgroup
+// points per workgroup (B lines of n): 8 per thread for complex128, 16 for complex64, so that a
+// workgroup's LDS is about the same and a strided axis reads 128 contiguous bytes (B adjacent lines)
+// (power-of-two lengths only: the mixed-radix kernel's register staging would spill)
+template <typename R, bool POW2>
+struct FftPts {
+  static constexpr int kMax = sizeof(R) == 4 && POW2 ? 8192 : 4096;
+};
 constexpr int kMaxStages = 24;
 
 template <typename R>
@@ -141,7 +148,7 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
 template <typename R, int RR, bool POW2 = false>
 __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restrict__ tw, int n, int B, int p) {
   const int LS = line_slots<R>(n);
-  constexpr int MAXB = (4096 / kFftThreads + RR - 1) / RR;   // B * n <= 4096: at most ceil(8/RR) butterflies per thread
+  constexpr int MAXB = (FftPts<R, POW2>::kMax / kFftThreads + RR - 1) / RR;   // butterflies per thread (B n <= kMax)
   const int m = n / RR;
   const int tstep = n / (p * RR);
   const int total = B * m;
@@ -255,8 +262,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
       a = POW2 ? e >> lgb : e / B;
     }
   };
-  // all of a thread's loads are issued before any is used (B n <= 4096: 8 per thread)
-  constexpr int kPer = 4096 / kFftThreads;
+  // all of a thread's loads are issued before any is used (B n <= kMax: 8 or 16 per thread)
+  constexpr int kPer = FftPts<R, POW2>::kMax / kFftThreads;
   Cx<R> ld[kPer];
 #pragma unroll
   for (int q = 0; q < kPer; ++q) {
@@ -512,12 +519,13 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
   bool inplace = true;
   for (int r : f) inplace = inplace && r <= 8;
   const int planes = inplace ? 1 : 2;
-  // B lines of n points per workgroup: B * n <= 4096 (the register staging bound of stage_inplace)
+  // B lines of n points per workgroup: B * n <= kMax (the register staging bound of stage_inplace)
+  const bool pow2 = (n & (n - 1)) == 0;
+  const int kMaxPts = pow2 ? FftPts<R, true>::kMax : FftPts<R, false>::kMax;
   QM_REQUIRE(ctx, n <= 4096, QM_ELIMIT, "fft: length %d exceeds 4096", n);
-  int B = std::max(1, std::min(16, 4096 / n));
+  int B = std::max(1, std::min(16, kMaxPts / n));
   if (!inplace) B = std::max(1, std::min(B, 2048 / n));
   B = std::max(1, std::min(B, ln.nl1));
-  const bool pow2 = (n & (n - 1)) == 0;
   if (pow2)
     while (B & (B - 1)) B &= B - 1;   // the power-of-two kernel splits indices by shifts
   ln.B = B;

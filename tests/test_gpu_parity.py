@@ -103,6 +103,23 @@ def test_density_mixed(qm, small, prec, tol):
     assert abs(q.raw_total - raw) / raw <= tol
 
 
+@pytest.mark.parametrize("natoms,seed", [(1, 3), (2, 5), (3, 7), (9, 2), (12, 4), (13, 8), (15, 9)])
+def test_density_block_alignment(qm, obasis, natoms, seed):
+    """nbf 14 .. 156: the default contractions right-align their 128-column blocks on nbf, so
+    block 0 holds 0 .. 112 padding columns below column 0 (and nbf = 128 none at all)."""
+    from oracle import density as od
+    from qmtools import synth
+    mol = synth.synthetic_molecule(natoms, seed, qm.BasisSet(BASIS))
+    grid = synth.centred_grid(mol, 16, step_bohr=0.5, pad_ang=2.0)
+    nx, ny, nz, h, o = grid.geometry()
+    _say("density block alignment", natoms, seed, "nbf", mol.nbf)
+    want, raw = od.density_grid(_system(mol, obasis), nx, ny, nz, h, o)
+    q = qm.QMTools.Compute_density(grid, mol)
+    assert _rel(q.qube, want) <= 1e-10
+    q = qm.QMTools.Compute_density(grid, mol, precision="mixed")
+    assert _rel(q.qube, want) <= 2e-6
+
+
 @pytest.mark.parametrize("prec,tol", [("f16", 2e-6), ("bf16", 1e-4)])
 @pytest.mark.parametrize("subgrid", [1, 2])
 @pytest.mark.parametrize("variant", [0, 1, 2, 3])

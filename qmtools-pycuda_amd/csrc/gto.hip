@@ -75,8 +75,10 @@ __device__ __forceinline__ void boys(double T, const double* __restrict__ tab, d
 #pragma unroll
     for (int n = L; n >= 1; --n) F[n - 1] = (2.0 * T * F[n] + ex) * kInvInt[2 * n - 1];
   } else {
-    const double inv = 1.0 / T;
-    F[0] = 0.5 * sqrt(M_PI * inv);
+    // F_0 = sqrt(pi / T) / 2 from one reciprocal square root (no FP64 division or sqrt), 1/T = r^2
+    const double r = rsqrt(T);
+    const double inv = r * r;
+    F[0] = 0.88622692545275801365 * r;   // sqrt(pi) / 2
 #pragma unroll
     for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * ((double)(2 * n - 1) * 0.5 * inv);
   }

@@ -629,6 +629,10 @@ __device__ __forceinline__ void lds_dma16(const void* gsrc, const void* lds_wave
   asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(l), "v"(gsrc) : "memory");
 }
 
+// k per stage of the DMA-fed FP64 tiles (ring of 32 / kF64Ks stages).  Stages of 8 k in a ring
+// of 4 (one more stage of lookahead, twice the barriers) measured 4% slower than 16 k in a ring of 2.
+constexpr int kF64Ks = 16;
+
 // Row-dot of the transposed FP64 tiles (M was the first MFMA operand): lane
 // (li, lk) register q of tile (r, c) is X[t = li][n = lk + 4 q]; 16 lanes read 16
 // consecutive points of one function (128 contiguous bytes); the 4 lane groups
@@ -727,10 +731,15 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
 // one k row, so a wave fills one padded LDS row) into a two-stage ring, issued
 // right after the barrier that opens the previous stage.  The loads come from
 // inline assembly (lds_dma16), the wait before each barrier is explicit.
+// KS k per stage in a ring of NB stages (NB KS = 32: the same 72 KB of LDS); stage st + NB - 1 is
+// requested right after the barrier that opens stage st.
+template <int KS, int NB>
 __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double* __restrict__ phi, int ld,
                                                    const double* __restrict__ M, int np, int nbf, int ncb,
                                                    double* __restrict__ part, int ldp) {
-  __shared__ __align__(16) double ring[2][2][kTKS][kTT + kTPad];
+  static_assert(KS % 4 == 0 && (2 * KS) % 8 == 0, "stages of whole MFMA k-steps, rows split over 8 waves");
+  constexpr int RPW = 2 * KS / 8;   // rows of one operand fetched per wave and stage
+  __shared__ __align__(16) double ring[NB][2][KS][kTT + kTPad];
   __shared__ double red[2][kTT];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -743,17 +752,17 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
   // padding, as columns below 0; rows and columns of M from nbf on are zero and are not visited.
   const int ne = (nbf + 15) / 16 * 16;
   const int row0 = rt * kTT, c0 = cb * kTT - (ncb * kTT - ne);
-  const int nst = (c0 + kTT) / kTKS;
+  const int nst = (c0 + kTT) / KS;
   const int cpad = min(max(-(c0 + 64 * wc) / 16, 0), 4);   // tile columns of this wave below column 0
-  // wave w brings k rows 4 (w % 4) .. 4 (w % 4) + 3 of operand w / 4 (0: basis values, 1: M)
-  const int op = wave >> 2, k0 = 4 * (wave & 3);
+  // wave w brings k rows RPW (w % 4) .. RPW (w % 4) + RPW - 1 of operand w / 4 (0: basis values, 1: M)
+  const int op = wave >> 2, k0 = RPW * (wave & 3);
   const double* src = op == 0 ? phi + row0 + 2 * lane : M + min(max(c0 + 2 * lane, 0), np - 2);   // clamped: unused
   const int64_t lds_ld = op == 0 ? ld : np;
   auto issue = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < RPW; ++j) {
       const int k = k0 + j;
-      lds_dma16(src + (int64_t)(st * kTKS + k) * lds_ld, &ring[st & 1][op][k][0]);
+      lds_dma16(src + (int64_t)(st * KS + k) * lds_ld, &ring[st % NB][op][k][0]);
     }
   };
   d4 acc[2][4];
@@ -761,21 +770,26 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = d4{0, 0, 0, 0};
-  issue(0);
+#pragma unroll
+  for (int q = 0; q < NB - 1; ++q)
+    if (q < nst) issue(q);
   for (int st = 0; st < nst; ++st) {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this thread's loads of stage st
-    __syncthreads();                                    // everyone's loads of st; everyone done with st - 1
-    if (st + 1 < nst) issue(st + 1);
-    const int buf = st & 1;
-    // M is zero below its diagonal: in the block's diagonal stages tile column c (columns
-    // c0 + 64 wc + 16 c ..) needs stage st only while 16 st <= c0 + 64 wc + 16 c, i.e.
-    // c >= st - c0 / 16 - 4 wc; tile columns below column 0 are never needed.  The first needed
-    // tile column is uniform per wave; each case is its own straight-line body so the
+    // this thread's loads of stage st are done once at most RPW (NB - 2) newer ones are pending
+    const int newer = min(NB - 2, nst - 1 - st);
+    if (RPW == 2 && newer >= 2) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+    else if (RPW == 2 && newer == 1) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();   // everyone's loads of st; everyone done with st - 1, whose slot is refilled next
+    if (st + NB - 1 < nst) issue(st + NB - 1);
+    const int buf = st % NB;
+    // M is zero below its diagonal: tile column c (columns n0 = c0 + 64 wc + 16 c ..) needs stage
+    // st only while KS st <= n0 + 15; tile columns below column 0 are never needed.  The first
+    // needed tile column is uniform per wave; each case is its own straight-line body so the
     // accumulators stay in place.
     auto body = [&](auto first) __attribute__((always_inline)) {
       constexpr int cs = decltype(first)::value;
 #pragma unroll
-      for (int ks = 0; ks < kTKS; ks += 4) {
+      for (int ks = 0; ks < KS; ks += 4) {
         double a[2], b[4];
 #pragma unroll
         for (int r = 0; r < 2; ++r) a[r] = ring[buf][0][ks + lk][wr * 32 + r * 16 + li];
@@ -788,7 +802,9 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
             acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c], a[r], acc[r][c], 0, 0, 0);
       }
     };
-    const int cs = max(min(max(st - c0 / kTKS - 4 * wc, 0), 4), cpad);
+    // first needed tile column: KS st <= c0 + 64 wc + 16 c + 15  <=>  c >= (KS st - c0 - 64 wc - 15) / 16
+    const int need = KS * st - c0 - 64 * wc - 15;
+    const int cs = max(min(need > 0 ? (need + 15) / 16 : 0, 4), cpad);
     // (a tile computed anyway only adds products with M's zeros, or, below column 0, results
     // the row-dot ignores, so cs is rounded down)
     if (cs < 2) body(std::integral_constant<int, 0>());
@@ -1205,7 +1221,7 @@ __global__ __launch_bounds__(kTWG) void k_contract_f64_tiles(const double* __res
                                                              int ncb, double* __restrict__ part) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  if (DMA) tiled_f64_dma_body(rt, cb, phi, ld, M, np, nbf, ncb, part, ld);
+  if (DMA) tiled_f64_dma_body<kF64Ks, 32 / kF64Ks>(rt, cb, phi, ld, M, np, nbf, ncb, part, ld);
   else tiled_f64_body(rt, cb, phi, ld, M, np, part, ld);
 }
 
@@ -2003,8 +2019,8 @@ __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __re
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
   if (DMA)
-    tiled_f64_dma_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.mol.nbf, q.ncb, q.part,
-                       q.ld);
+    tiled_f64_dma_body<kF64Ks, 32 / kF64Ks>(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad,
+                                            q.mol.nbf, q.ncb, q.part, q.ld);
   else tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 

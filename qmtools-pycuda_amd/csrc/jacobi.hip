@@ -103,7 +103,7 @@ __device__ __forceinline__ void strip_body(int bx, int by, int bz, int zs, const
       sum += pc[r];
       sum += pp[r + 1];
       sum += pm[r + 1];
-      sum -= rr[r] * hinv;
+      sum = fma(-rr[r], hinv, sum);   // explicit: every copy of this code rounds alike
       const bool outer = zb || i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
       const T val = outer ? c : sum * (T)(1.0 / 6.0);
       if (i < nx && j < ny) vout[pk + sy * (size_t)j + (unsigned)i] = val;
@@ -168,40 +168,53 @@ __device__ __forceinline__ void strip2_body(int bx, int by, int bz, int zs, cons
   T am[R0], ac[R0], ap[R0];         // V on planes m-1, m, m+1, rows j0-2+q
   T bmm[R1], bm[R1];                // first sweep on planes m-2, m-1, rows j0-1+p
   T hx[R1], rc[R1], rm[R1];         // halo column of plane m; rho of planes m and m-1 (rows j0-1+p)
+  // Loads run two planes ahead, into ping-pong buffers: iteration m (parity b) requests V
+  // plane m+3, rho and halo of plane m+2 into buffer b and moves the data requested one
+  // iteration earlier (buffer 1-b) into place, so no wave waits for loads it has just issued.
+  T an[2][R0], rn[2][R1], hn[2][R1];
   const int m0 = z0 - 1;
+  auto request = [&](int m, auto buf) __attribute__((always_inline)) {
+    constexpr int b = decltype(buf)::value;
+    const size_t pv = pln(m + 3), pr = pln(m + 2);
+#pragma unroll
+    for (int q = 0; q < R0; ++q) an[b][q] = v[pv + row(j0 - 2 + q) + ic];
+#pragma unroll
+    for (int p = 0; p < R1; ++p) {
+      rn[b][p] = rho[pr + row(j0 - 1 + p) + ic];
+      hn[b][p] = (T)0;
+    }
+    if (edge) {
+#pragma unroll
+      for (int p = 0; p < R1; ++p) hn[b][p] = v[pr + row(j0 - 1 + p) + ihc];
+    }
+  };
 #pragma unroll
   for (int q = 0; q < R0; ++q) {
     am[q] = v[pln(m0 - 1) + row(j0 - 2 + q) + ic];
     ac[q] = v[pln(m0) + row(j0 - 2 + q) + ic];
     ap[q] = v[pln(m0 + 1) + row(j0 - 2 + q) + ic];
+    an[1][q] = v[pln(m0 + 2) + row(j0 - 2 + q) + ic];
   }
 #pragma unroll
   for (int p = 0; p < R1; ++p) {
     rc[p] = rho[pln(m0) + row(j0 - 1 + p) + ic];
+    rn[1][p] = rho[pln(m0 + 1) + row(j0 - 1 + p) + ic];
     rm[p] = (T)0;
     hx[p] = (T)0;
+    hn[1][p] = (T)0;
     bmm[p] = (T)0;
     bm[p] = (T)0;
   }
   if (edge) {
 #pragma unroll
-    for (int p = 0; p < R1; ++p) hx[p] = v[pln(m0) + row(j0 - 1 + p) + ihc];
-  }
-  for (int m = m0; m <= z1; ++m) {
-    // prefetch: V plane m+2, halo and rho of plane m+1
-    T an[R0], hn[R1], rn[R1];
-    const size_t pm2 = pln(m + 2), pm1 = pln(m + 1);
-#pragma unroll
-    for (int q = 0; q < R0; ++q) an[q] = v[pm2 + row(j0 - 2 + q) + ic];
-#pragma unroll
     for (int p = 0; p < R1; ++p) {
-      rn[p] = rho[pm1 + row(j0 - 1 + p) + ic];
-      hn[p] = (T)0;
+      hx[p] = v[pln(m0) + row(j0 - 1 + p) + ihc];
+      hn[1][p] = v[pln(m0 + 1) + row(j0 - 1 + p) + ihc];
     }
-    if (edge) {
-#pragma unroll
-      for (int p = 0; p < R1; ++p) hn[p] = v[pm1 + row(j0 - 1 + p) + ihc];
-    }
+  }
+  auto step = [&](int m, auto buf) __attribute__((always_inline)) {
+    constexpr int b = decltype(buf)::value;
+    request(m, buf);
     // first sweep on plane m
     const bool zb1 = m <= 0 || m >= nz - 1;
     T bc[R1];
@@ -218,7 +231,7 @@ __device__ __forceinline__ void strip2_body(int bx, int by, int bz, int zs, cons
       sum += ac[p];
       sum += ap[p + 1];
       sum += am[p + 1];
-      sum -= rc[p] * hinv;
+      sum = fma(-rc[p], hinv, sum);   // explicit: every copy of this code rounds alike
       const bool outer = zb1 || xb || j <= 0 || j >= ny - 1;
       bc[p] = outer ? c : sum * (T)(1.0 / 6.0);
     }
@@ -237,7 +250,7 @@ __device__ __forceinline__ void strip2_body(int bx, int by, int bz, int zs, cons
         sum += bm[r];
         sum += bc[r + 1];
         sum += bmm[r + 1];
-        sum -= rm[r + 1] * hinv;
+        sum = fma(-rm[r + 1], hinv, sum);
         const bool outer = zb2 || xb || j == 0 || j == ny - 1;
         const T val = outer ? c : sum * (T)(1.0 / 6.0);
         if (lane >= 1 && lane <= kSW - 2 && i < nx && j < ny) vout[sz * (size_t)k + sy * (size_t)j + (unsigned)i] = val;
@@ -247,17 +260,23 @@ __device__ __forceinline__ void strip2_body(int bx, int by, int bz, int zs, cons
     for (int q = 0; q < R0; ++q) {
       am[q] = ac[q];
       ac[q] = ap[q];
-      ap[q] = an[q];
+      ap[q] = an[1 - b][q];
     }
 #pragma unroll
     for (int p = 0; p < R1; ++p) {
       bmm[p] = bm[p];
       bm[p] = bc[p];
       rm[p] = rc[p];
-      rc[p] = rn[p];
-      hx[p] = hn[p];
+      rc[p] = rn[1 - b][p];
+      hx[p] = hn[1 - b][p];
     }
+  };
+  int m = m0;
+  for (; m + 1 <= z1; m += 2) {
+    step(m, std::integral_constant<int, 0>());
+    step(m + 1, std::integral_constant<int, 1>());
   }
+  if (m <= z1) step(m, std::integral_constant<int, 0>());
 }
 
 constexpr int kR2D = 4, kR2F = 8;   // rows per strip of the two-sweep kernel: FP64, FP32
@@ -271,12 +290,15 @@ __global__ __launch_bounds__(kSW* kSWaves) void k_sweep_strip2(const T* __restri
   strip2_body<T, r2<T>()>(blockIdx.x, blockIdx.y, blockIdx.z, zs, rho, v, vout, nx, ny, nz, h);
 }
 
+// planes per workgroup of the two-sweep kernel: at most 1024 workgroups (two rounds of the
+// 512 that fit the chip at two waves per SIMD), at least 4 planes each; a positive `fixed`
+// (the tuning knob) overrides
 template <typename T>
 inline int pick_zs2(int nx, int ny, int nz, int fixed) {
   if (fixed > 0) return std::min(fixed, nz);
   const int tiles = ((nx + Strip2<r2<T>()>::W - 1) / Strip2<r2<T>()>::W) *
                     ((ny + Strip2<r2<T>()>::TY - 1) / Strip2<r2<T>()>::TY);
-  const int tz = std::max(1, (1024 + tiles - 1) / tiles);
+  const int tz = std::max(1, 1024 / tiles);
   return std::max(4, (nz + tz - 1) / tz);
 }
 
@@ -879,8 +901,17 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
   std::vector<JItem> items(n);
   int blocks = 0, sblocks = 0, tblocks = 0;
   size_t f32_elems = 0;
+  constexpr int RD = kR2D, RF = kR2F;
+  const int W = Strip2<RD>::W, TY = f64 ? Strip2<RD>::TY : Strip2<RF>::TY;
+  // two-sweep passes: one slab length for the whole batch, for about 1024 workgroups in all
+  // (per-grid slabs of small grids would be a few planes, each re-reading 4 halo planes)
+  int64_t tile_planes = 0;
   for (int i = 0; i < n; ++i) {
     if ((rc = qm_check_geom(ctx, &geoms[i]))) return rc;
+    tile_planes += (int64_t)((geoms[i].nx + W - 1) / W) * ((geoms[i].ny + TY - 1) / TY) * geoms[i].nz;
+  }
+  const int zs2_batch = ctx->jacobi_zs > 0 ? ctx->jacobi_zs : (int)std::max<int64_t>(4, (tile_planes + 1023) / 1024);
+  for (int i = 0; i < n; ++i) {
     QM_REQUIRE(ctx, rho[i] && v[i] && (!f64 || (tmp && tmp[i])), QM_EINVAL, "jacobi_batch: item %d incomplete", i);
     JItem& q = items[i];
     q.nx = geoms[i].nx;
@@ -896,9 +927,7 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
     q.blk0s = sblocks;
     sblocks += q.tbx * q.tby * ((q.nz + q.zs - 1) / q.zs);
     {
-      constexpr int RD = kR2D, RF = kR2F;
-      const int W = Strip2<RD>::W, TY = f64 ? Strip2<RD>::TY : Strip2<RF>::TY;
-      q.zs2 = f64 ? pick_zs2<double>(q.nx, q.ny, q.nz, ctx->jacobi_zs) : pick_zs2<float>(q.nx, q.ny, q.nz, ctx->jacobi_zs);
+      q.zs2 = std::min(zs2_batch, q.nz);
       q.tbx2 = (q.nx + W - 1) / W;
       q.tby2 = (q.ny + TY - 1) / TY;
       q.blk0t = tblocks;

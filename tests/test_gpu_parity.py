@@ -213,6 +213,38 @@ def test_jacobi_ragged_grid(qm, precision, sweeps):
 
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_jacobi_slab_size_invariant(qm, precision):
+    """Sweeps on 64 x 40 x 61 are bitwise the same for every z slab per workgroup: odd and even
+    plane counts take different unrolled copies of the two-sweep loop, which must round alike."""
+    import ctypes
+    import torch
+    from qmtools import _native as nat, autotune
+    nx, ny, nz, h, sweeps = 64, 40, 61, 0.25, 46
+    rng = numpy.random.default_rng(11)
+    rho = torch.tensor(rng.standard_normal(nx * ny * nz), device="cuda")
+    g = nat.geom((nx, ny, nz, h, (0.0, 0.0, 0.0)))
+    old = autotune.get("jacobi_zs")
+    outs = {}
+    try:
+        for zs in (0, 4, 5, 7, 13, 61):
+            _say("jacobi slab size", precision, zs)
+            autotune.set("jacobi_zs", zs)
+            dv = torch.zeros_like(rho)
+            dt = torch.zeros_like(rho)
+            done, delta = ctypes.c_int64(0), ctypes.c_double(0)
+            if precision == "fp64":
+                args = (nat.QM_PREC_FP64, nat.dptr(rho), nat.dptr(dv), nat.dptr(dt), sweeps, -1.0, 0)
+            else:
+                args = (nat.QM_PREC_FP32, nat.dptr(rho), nat.dptr(dv), None, sweeps, -1.0, 50)
+            nat.call("qm_jacobi", nat.stream(), ctypes.byref(g), *args, ctypes.byref(done), ctypes.byref(delta))
+            outs[zs] = dv.cpu().numpy()
+    finally:
+        autotune.set("jacobi_zs", old)
+    for zs, v in outs.items():
+        numpy.testing.assert_array_equal(v, outs[0], err_msg=f"z slab {zs}")
+
+
 @pytest.mark.parametrize("boundary", ["zero", "multipole"])
 def test_jacobi_fp64_fixed_sweeps(qm, small, boundary):
     from oracle import density as od

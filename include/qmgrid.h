@@ -253,7 +253,8 @@ enum {
                                loads of the dense P, no barriers, 2 = 128 x 128 tiles with per-block partial sums,
                                3 = 128 x 64 tiles with diagonal skipping, 4 = 128 x 128 tiles fed by LDS DMA with
                                diagonal skipping (default) */
-  QM_TUNE_JACOBI_ZS = 2,    /* z planes per Jacobi sweep workgroup, 0 = chosen per grid, else 1..4096 */
+  QM_TUNE_JACOBI_ZS = 2,    /* z planes per Jacobi sweep workgroup, 0 = chosen per grid (the two-sweep passes
+                               of qm_jacobi_batch: one length for the whole batch), else 1..4096 */
   QM_TUNE_COUNT = 3
 };
 int qm_tune_set(qm_ctx* ctx, int knob, int value);

@@ -312,9 +312,15 @@ inline int pick_zs(int nx, int ny, int nz, int fixed) {
 }
 
 // r = rho - h (sum_6 V - 6 V) on interior points (FP64), 0 on the outer layer, stored FP32
+// The first FP32 sweep of the correction from e = 0: every neighbour sum is +0, so the
+// sweep's value is fma(-r, 1/h, +0) * (1/6) on interior points and 0 on the outer layer,
+// bitwise what the sweep kernels compute (hinv is the same rounded 1/h).
+__device__ __forceinline__ float first_correction(float r, float hinv) { return fmaf(-r, hinv, 0.f) * (float)(1.0 / 6.0); }
+
+// e1 (optional): also store the first sweep of the correction
 __global__ __launch_bounds__(kBX* kBY) void k_residual(const double* __restrict__ rho,
                                                        const double* __restrict__ v, float* __restrict__ r,
-                                                       int nx, int ny, int nz, double h) {
+                                                       float* __restrict__ e1, int nx, int ny, int nz, double h) {
   const int i = blockIdx.x * kBX + threadIdx.x;
   const int j = blockIdx.y * kBY + threadIdx.y;
   const int k = blockIdx.z;
@@ -323,6 +329,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_residual(const double* __restrict_
   const int64_t idx = i + sy * j + sz * k;
   if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) {
     r[idx] = 0.f;
+    if (e1) e1[idx] = 0.f;
     return;
   }
   const double c = v[idx];
@@ -331,7 +338,9 @@ __global__ __launch_bounds__(kBX* kBY) void k_residual(const double* __restrict_
   s += v[idx - sy];
   s += v[idx + sz];
   s += v[idx - sz];
-  r[idx] = (float)(rho[idx] - h * (s - 6.0 * c));
+  const float rr = (float)(rho[idx] - h * (s - 6.0 * c));
+  r[idx] = rr;
+  if (e1) e1[idx] = first_correction(rr, 1.0f / (float)h);   // 1/h rounded as in the sweep kernels
 }
 
 __global__ void k_add_correction(double* __restrict__ v, const float* __restrict__ e, int64_t n) {
@@ -602,15 +611,27 @@ int jacobi_fp32(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho,
   if ((rc = qm_ws(ctx, WS_JAC_B, sizeof(float) * n, (void**)&e))) return rc;
   if ((rc = qm_ws(ctx, WS_JAC_C, sizeof(float) * n, (void**)&t))) return rc;
   const unsigned upd_blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
-  // one residual block of m sweeps: residual, clear, m FP32 sweeps, correction (kernels only)
+  const bool checks = tol >= 0;
+  // one residual block of m sweeps: residual, m FP32 sweeps, correction (kernels only).
+  // With convergence checks the correction starts from cleared buffers and the block ends in
+  // two single sweeps (e last, t the one before); without, the residual kernel also forms the
+  // first sweep (into t) and the other m - 1 run as two-sweep passes (a single one if odd).
   auto enqueue = [&](hipStream_t q, int64_t m, bool prof) -> int {
-    hipLaunchKernelGGL(k_residual, sweep_grid(g), dim3(kBX, kBY), 0, q, rho, (const double*)v, r, g->nx, g->ny,
-                       g->nz, g->step);
-    hipLaunchKernelGGL(k_fill0, dim3(upd_blocks), dim3(256), 0, q, e, t, n);
     float* a = e;
     float* b = t;
     int64_t P, S;
-    split_block(m, P, S);
+    if (checks) {
+      hipLaunchKernelGGL(k_residual, sweep_grid(g), dim3(kBX, kBY), 0, q, rho, (const double*)v, r, nullptr, g->nx,
+                         g->ny, g->nz, g->step);
+      hipLaunchKernelGGL(k_fill0, dim3(upd_blocks), dim3(256), 0, q, e, t, n);
+      split_block(m, P, S);
+    } else {
+      hipLaunchKernelGGL(k_residual, sweep_grid(g), dim3(kBX, kBY), 0, q, rho, (const double*)v, r, t, g->nx,
+                         g->ny, g->nz, g->step);
+      std::swap(a, b);
+      P = (m - 1) / 2;
+      S = (m - 1) % 2;
+    }
     for (int64_t j = 0; j < P; ++j) {
       int r1 = sweep2<float>(ctx, q, g, r, a, b, prof);
       if (r1) return r1;
@@ -627,7 +648,6 @@ int jacobi_fp32(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho,
   int64_t done = 0;
   double delta = -1.0;
   bool converged = false;
-  const bool checks = tol >= 0;
   if (max_sweeps >= block) {
     GraphBlock gb(ctx, s);
     if ((rc = gb.capture([&](hipStream_t q) { return enqueue(q, block, false); }))) return rc;
@@ -831,12 +851,11 @@ __global__ __launch_bounds__(kBX* kBY) void kb_residual(const JItem* __restrict_
   if (!item_point(q, blockIdx.x - q.blk0, i, j, k)) return;
   const int64_t sy = q.nx, sz = (int64_t)q.nx * q.ny;
   const int64_t idx = i + sy * j + sz * k;
-  float* e = (float*)q.a;
+  // the first sweep of the correction goes to b (every point, so a needs no clearing)
   float* t = (float*)q.b;
-  e[idx] = 0.f;
-  t[idx] = 0.f;
   if (i == 0 || j == 0 || k == 0 || i == q.nx - 1 || j == q.ny - 1 || k == q.nz - 1) {
     q.r32[idx] = 0.f;
+    t[idx] = 0.f;
     return;
   }
   const double* v = q.v64;
@@ -846,7 +865,9 @@ __global__ __launch_bounds__(kBX* kBY) void kb_residual(const JItem* __restrict_
   s += v[idx - sy];
   s += v[idx + sz];
   s += v[idx - sz];
-  q.r32[idx] = (float)(q.rho64[idx] - q.h * (s - 6.0 * c));
+  const float rr = (float)(q.rho64[idx] - q.h * (s - 6.0 * c));
+  q.r32[idx] = rr;
+  t[idx] = first_correction(rr, 1.0f / (float)q.h);   // 1/h rounded as in the sweep kernels
 }
 
 __global__ __launch_bounds__(kBX* kBY) void kb_add(const JItem* __restrict__ items, int n, int from_b) {
@@ -980,10 +1001,12 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
       }
       if (dir) hipLaunchKernelGGL(kb_copy_back, grid, thr, 0, t, dt, n);
     } else {
+      // the residual kernel forms the first sweep of the correction (into b); kf - 1 follow
       hipLaunchKernelGGL(kb_residual, grid, thr, 0, t, dt, n);
-      for (int64_t q = 0; q < kf / 2; ++q, dir ^= 1)
+      dir = 1;
+      for (int64_t q = 0; q < (kf - 1) / 2; ++q, dir ^= 1)
         hipLaunchKernelGGL(kb_sweep2<float>, tgrid, sthr, 0, t, dt, n, dir);
-      if (kf & 1) {
+      if ((kf - 1) & 1) {
         hipLaunchKernelGGL(kb_sweep<float>, sgrid, sthr, 0, t, dt, n, dir);
         dir ^= 1;
       }

@@ -183,7 +183,8 @@ def test_vne_classic(qm, small):
 
 
 # ---------------------------------------------------------------- Jacobi
-@pytest.mark.parametrize("precision,sweeps", [("fp64", 250), ("fp64", 7), ("fp32", 250), ("fp32", 53)])
+@pytest.mark.parametrize("precision,sweeps", [("fp64", 250), ("fp64", 7), ("fp32", 250), ("fp32", 53),
+                                              ("fp32", 1), ("fp32", 2)])
 def test_jacobi_ragged_grid(qm, precision, sweeps):
     """qm_jacobi on a 70 x 13 x 9 grid (no strip, tile or slab multiple) with a held nonzero outer layer:
     two-sweep passes, single sweeps and graph replays against the oracle's plain sweeps."""
@@ -243,6 +244,36 @@ def test_jacobi_slab_size_invariant(qm, precision):
         autotune.set("jacobi_zs", old)
     for zs, v in outs.items():
         numpy.testing.assert_array_equal(v, outs[0], err_msg=f"z slab {zs}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_jacobi_checked_equals_unchecked(qm, precision):
+    """Blocks with convergence checks (each ends in two single sweeps; FP32 clears its correction
+    buffers) and blocks without (two-sweep passes; FP32 forms the first correction sweep in the
+    residual kernel) give bitwise the same V after the same 157 sweeps (3 blocks + 7)."""
+    import ctypes
+    import torch
+    from qmtools import _native as nat
+    nx, ny, nz, h, sweeps = 40, 48, 33, 0.2, 157
+    rng = numpy.random.default_rng(23)
+    rho = torch.tensor(rng.standard_normal(nx * ny * nz), device="cuda")
+    g = nat.geom((nx, ny, nz, h, (0.0, 0.0, 0.0)))
+    lib, ctx = nat.library(), nat.context()
+    outs = []
+    for tol in (-1.0, 1e-300):
+        _say("jacobi checked vs unchecked", precision, tol)
+        dv = torch.zeros_like(rho)
+        dt = torch.zeros_like(rho)
+        done, delta = ctypes.c_int64(0), ctypes.c_double(0)
+        if precision == "fp64":
+            args = (nat.QM_PREC_FP64, nat.dptr(rho), nat.dptr(dv), nat.dptr(dt), sweeps, tol, 50)
+        else:
+            args = (nat.QM_PREC_FP32, nat.dptr(rho), nat.dptr(dv), None, sweeps, tol, 50)
+        rc = lib.qm_jacobi(ctx, nat.stream(), ctypes.byref(g), *args, ctypes.byref(done), ctypes.byref(delta))
+        assert rc == (nat.QM_OK if tol < 0 else nat.QM_ENOCONV), rc   # 1e-300 is never reached
+        assert done.value == sweeps
+        outs.append(dv.cpu().numpy())
+    numpy.testing.assert_array_equal(outs[1], outs[0])
 
 
 @pytest.mark.parametrize("boundary", ["zero", "multipole"])

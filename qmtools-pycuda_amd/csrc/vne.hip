@@ -194,11 +194,22 @@ __global__ __launch_bounds__(256) void k_vne_sweep(const T* __restrict__ v, T* _
   };
   read_block(pl[(z0 - 1) & 3], blk[0]);
   read_block(pl[z0 & 3], blk[1]);
-  for (int k = z0; k < z1; ++k) {
-    T nxt[kVLoads];
-    const int64_t pk2 = plane_at(k + 2);
+  // Plane loads run two planes ahead into ping-pong registers: iteration k (parity b) requests
+  // plane k+3 into nxt[b] and stores plane k+2, requested one iteration earlier, into its LDS
+  // slot, so no thread waits for loads it has just issued.  The loads are unconditional (threads
+  // past the tile's last slot read a clamped address and store nothing): a load under a branch
+  // makes the compiler wait for every outstanding load before the LDS store.
+  T nxt[2][kVLoads];
+  {
+    const int64_t pk2 = plane_at(z0 + 2);
 #pragma unroll
-    for (int u = 0; u < kVLoads; ++u) nxt[u] = slot_e[u] < kVPlane ? v[pk2 + off[u]] : (T)0;
+    for (int u = 0; u < kVLoads; ++u) nxt[1][u] = v[pk2 + off[u]];
+  }
+  auto step = [&](int k, auto buf) __attribute__((always_inline)) {
+    constexpr int b = decltype(buf)::value;
+    const int64_t pk3 = plane_at(k + 3);
+#pragma unroll
+    for (int u = 0; u < kVLoads; ++u) nxt[b][u] = v[pk3 + off[u]];   // off[u] is clamped: always in the grid
     read_block(pl[(k + 1) & 3], blk[2]);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -214,16 +225,17 @@ __global__ __launch_bounds__(256) void k_vne_sweep(const T* __restrict__ v, T* _
             const int d = (x != 0) + (y != 0) + (z != 0);
             if (d == 0) continue;
             const double w = d == 1 ? w1 : (d == 2 ? w2 : w3);
-            sum += w * (double)blk[z + 1][q + 1 + y][1 + x];
+            sum = fma(w, (double)blk[z + 1][q + 1 + y][1 + x], sum);
           }
-      sum += self * c;
-      sum -= c * ads;
+      // multiply-adds written as FMAs: the unrolled copies of this body round alike
+      sum = fma(self, c, sum);
+      sum = fma(-c, ads, sum);
       const double px = (gx + 0.5) * h, py = (gy + 0.5) * h, pz = (k + 0.5) * h;
       for (int idx = 0; idx < nlist; ++idx) {
         const double4 A = atoms[all_atoms ? idx : near[idx]];
         const double ex = px - (A.x - g.ox), ey = py - (A.y - g.oy), ez = pz - (A.z - g.oz);
-        const double d = sqrt(ex * ex + ey * ey + ez * ez);
-        if (d < rcut) sum += A.w * exp(-d);
+        const double d = sqrt(fma(ex, ex, fma(ey, ey, ez * ez)));
+        if (d < rcut) sum = fma(A.w, exp(-d), sum);
       }
       if (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1 || k == 0 || k == nz - 1) sum = 0.0;
       if (gx < nx && gy < ny) {
@@ -235,7 +247,7 @@ __global__ __launch_bounds__(256) void k_vne_sweep(const T* __restrict__ v, T* _
     // plane k+2 goes to the slot of plane k-2, last read before the previous barrier
 #pragma unroll
     for (int u = 0; u < kVLoads; ++u)
-      if (slot_e[u] < kVPlane) pl[(k + 2) & 3][slot_e[u]] = nxt[u];
+      if (slot_e[u] < kVPlane) pl[(k + 2) & 3][slot_e[u]] = nxt[1 - b][u];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -244,7 +256,13 @@ __global__ __launch_bounds__(256) void k_vne_sweep(const T* __restrict__ v, T* _
         blk[1][r][c] = blk[2][r][c];
       }
     __syncthreads();
+  };
+  int k = z0;
+  for (; k + 1 < z1; k += 2) {
+    step(k, std::integral_constant<int, 0>());
+    step(k + 1, std::integral_constant<int, 1>());
   }
+  if (k < z1) step(k, std::integral_constant<int, 0>());
   if (part) {
     acc = vne_wave_sum(acc);
     if (tx == 0) red[ty] = acc;

@@ -738,6 +738,7 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
                                                    const double* __restrict__ M, int np, int nbf, int ncb,
                                                    double* __restrict__ part, int ldp) {
   static_assert(KS % 4 == 0 && (2 * KS) % 8 == 0, "stages of whole MFMA k-steps, rows split over 8 waves");
+  static_assert(KS == 16, "the row-dot below takes one 16-column tile per stage");
   constexpr int RPW = 2 * KS / 8;   // rows of one operand fetched per wave and stage
   __shared__ __align__(16) double ring[NB][2][KS][kTT + kTPad];
   __shared__ double red[2][kTT];
@@ -766,6 +767,7 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
     }
   };
   d4 acc[2][4];
+  double pr[2] = {0.0, 0.0};   // this lane's row-dot partial sums (rows wr 32 + 16 r + li)
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -809,8 +811,32 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
     // the row-dot ignores, so cs is rounded down)
     if (cs < 2) body(std::integral_constant<int, 0>());
     else if (cs < 4) body(std::integral_constant<int, 2>());
+    // Row-dot of the tile this stage completes: tile c (columns n0 = c0 + 64 wc + 16 c ..) gets its
+    // last products in the stage with KS st = n0, whose A operand in the ring slot holds the basis
+    // values of exactly its columns (k = n - n0 = lk + 4 q for lane (li, lk), register q).  Tiles
+    // are summed in column order, as f64_rowdot does from global memory.
+    auto rowdot = [&](auto tile) __attribute__((always_inline)) {
+      constexpr int c = decltype(tile)::value;
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pr[r] = fma(acc[r][c][q], ring[buf][0][lk + 4 * q][wr * 32 + r * 16 + li], pr[r]);
+    };
+    const int cfin = (KS * st - c0 - 64 * wc) / 16;   // exact: c0 is a multiple of 16
+    if (cfin >= cpad) {                                // tiles below column 0 are never summed
+      if (cfin == 0) rowdot(std::integral_constant<int, 0>());
+      else if (cfin == 1) rowdot(std::integral_constant<int, 1>());
+      else if (cfin == 2) rowdot(std::integral_constant<int, 2>());
+      else if (cfin == 3) rowdot(std::integral_constant<int, 3>());
+    }
   }
-  f64_rowdot(acc, phi, ld, np, row0, c0, wr, wc, li, lk, red);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    double p = pr[r];
+    p += __shfl_xor(p, 16);
+    p += __shfl_xor(p, 32);
+    if (lk == 0) red[wc][wr * 32 + r * 16 + li] = p;
+  }
   __syncthreads();
   if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
 }
@@ -1062,10 +1088,13 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
     lds_dma16(p_lo + ob, &ring[slot][3][0][0] + 64 * wave);
   };
   f4 acc[RT][CT];
+  double pr[RT];   // this lane's row-dot partial sums (rows wr TRW + 16 r + li)
 #pragma unroll
-  for (int r = 0; r < RT; ++r)
+  for (int r = 0; r < RT; ++r) {
+    pr[r] = 0.0;
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[r][c] = f4{0, 0, 0, 0};
+  }
   issue(0);
   for (int st = 0; st < nst; ++st) {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this thread's loads of stage st
@@ -1097,11 +1126,44 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
           acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
         }
     };
-    const int cs = max(min(max(2 * (st - c0 / 32) - 4 * wc, 0), 4), cpad);
+    const int cd = 2 * (st - c0 / 32) - 4 * wc;   // first of the two tiles whose last stage this is
+    const int cs = max(min(max(cd, 0), 4), cpad);
     if (cs < 2) body(std::integral_constant<int, 0>());
     else if (cs < 4) body(std::integral_constant<int, 2>());
+    // Row-dot of the tiles this stage completes: tile c (columns n0 = c0 + 64 wc + 16 c ..) gets its
+    // last products in stage (n0 + 15) / 32, whose A operand in the ring slot holds the basis
+    // values of exactly its columns (k = n).  Lane (li, lk), register q of a tile is X[t = li][n =
+    // 4 lk + q]: the 4 values of one point sit in one 8-byte word of each plane.  Tiles are summed
+    // in column order, as the epilogue of tiled_split_body<PRE> does from global memory.
+    auto rowdot = [&](auto first) __attribute__((always_inline)) {
+      constexpr int c1 = decltype(first)::value;
+      typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+      typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+      typedef typename std::conditional<std::is_same<T16, _Float16>::value, h4, b4>::type V4;
+#pragma unroll
+      for (int dc = 0; dc < 2; ++dc) {
+        const int kg = 2 * dc + (lk >> 1);   // k group of n - 32 st = 16 dc + 4 lk + q
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+          const int row = wr * TRW + r * 16 + li;
+          const V4 h = __builtin_bit_cast(V4, ((const uint2*)&ring[slot][0][kg][row])[lk & 1]);
+          const V4 l = __builtin_bit_cast(V4, ((const uint2*)&ring[slot][1][kg][row])[lk & 1]);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) pr[r] += (double)(acc[r][c1 + dc][q] * ((float)h[q] + (float)l[q]));
+        }
+      }
+    };
+    // cpad is 0, 2 or 4 (c0 is a multiple of 32): a pair of tiles is all padding or none
+    if (cd == 0 && cpad == 0) rowdot(std::integral_constant<int, 0>());
+    else if (cd == 2 && cpad <= 2) rowdot(std::integral_constant<int, 2>());
   }
-  presplit_rowdot<T16, RT, CT, TRW, TCW, WC>(acc, a_hi, a_lo, ld, np, row0, c0, wr, wc, li, lk, red);
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    double p = pr[r];
+    p += __shfl_xor(p, 16);
+    p += __shfl_xor(p, 32);
+    if (lk == 0) red[wc][wr * TRW + r * 16 + li] = p;
+  }
   __syncthreads();
   if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
 }

@@ -37,12 +37,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per contraction launch measured with rocprofv3 FETCH_SIZE (x2 calibration) and
 # WRITE_SIZE (x1) for this workload; source: profiles/config2_pmc_traffic.md
 # (precision, nbf_pad, variant): (points per launch, HBM bytes per launch)
-MEASURED_TRAFFIC = {("fp64", 640, "tiles_dma"): (838784, 5527968 * 1024 * 2 + 32765 * 1024),
+MEASURED_TRAFFIC = {("fp64", 640, "tiles_dma"): (838784, 3787436 * 1024 * 2 + 32765 * 1024),
                     ("fp64", 640, "tiles"): (838784, 6138183 * 1024 * 2 + 32765 * 1024),
                     ("fp64", 640, "rows"): (838784, 3870778 * 1024 * 2 + 65536 * 1024),
                     ("fp64", 640, "lds"): (838784, 4444220 * 1024 * 2 + 190037 * 1024),
                     ("fp64", 640, "direct"): (838784, 7883104 * 1024 * 2 + 6553 * 1024),
-                    ("mixed", 640, 3): (1677696, 2053560 * 1024 * 2 + 65535 * 1024),
+                    ("mixed", 640, 3): (1677696, 1995799 * 1024 * 2 + 65535 * 1024),
                     ("mixed", 640, 1): (1677696, 2246457 * 1024 * 2 + 65535 * 1024),
                     ("mixed", 640, 0): (1677696, 2397406 * 1024 * 2 + 65535 * 1024),
                     ("mixed", 640, 2): (1677696, 2526254 * 1024 * 2 + 65535 * 1024),

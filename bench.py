@@ -419,6 +419,13 @@ def main():
                                                   "gto": gm / 2}
             other_out["contraction_avg_ms"] = km / max(kn, 1)
             other_out["contraction_launches_per_step"] = kn / 2
+            # the split contraction on the FP16 MFMAs: 3 products of the triangular form per point
+            nbf_m = w.mol.nbf
+            pts_m = w.N / max(kn / 2, 1)
+            ach_m = 3.0 * (nbf_m * (nbf_m + 1.0) + 2.0 * nbf_m) * pts_m / (km / max(kn, 1) * 1e-3) / 1e12
+            other_out["contraction_roofline"] = {"bound": "mfma", "achieved": ach_m, "peak": F16_MFMA_PEAK_TFLOPS,
+                                                 "unit": "TFLOP/s", "frac": ach_m / F16_MFMA_PEAK_TFLOPS,
+                                                 "algorithmic": "3 x (nbf (nbf + 1) + 2 nbf) FP16 MFMA flops/point"}
             log(f"mixed breakdown: {other_out['breakdown_ms_per_step']}")
         w.q64.copy_(q64)
 

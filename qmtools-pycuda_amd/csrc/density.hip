@@ -732,7 +732,8 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
 // right after the barrier that opens the previous stage.  The loads come from
 // inline assembly (lds_dma16), the wait before each barrier is explicit.
 // KS k per stage in a ring of NB stages (NB KS = 32: the same 72 KB of LDS); stage st + NB - 1 is
-// requested right after the barrier that opens stage st.
+// requested right after the barrier that opens stage st.  The row-dot is summed in the K loop from
+// the LDS copy of the basis values of each diagonal stage (below), not re-read from global memory.
 template <int KS, int NB>
 __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double* __restrict__ phi, int ld,
                                                    const double* __restrict__ M, int np, int nbf, int ncb,
@@ -1051,8 +1052,9 @@ __global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const voi
 // finished once it passes stage st's barrier.  The loads are issued from inline
 // assembly (M0 holds the LDS address) so that the compiler does not wait for
 // them before each LDS read; the explicit s_waitcnt vmcnt before the barrier
-// waits for them instead.  No other code in this kernel uses M0.  Tiles, the
-// transposed MFMA order and the row-dot are those of tiled_split_body<PRE>.
+// waits for them instead.  No other code in this kernel uses M0.  Tiles and the
+// transposed MFMA order are those of tiled_split_body<PRE>; its row-dot sums are
+// formed here in the K loop, from the LDS copy of the basis values (below).
 
 template <typename T16>
 __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __restrict__ a_hi, int ld,

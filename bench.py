@@ -266,29 +266,14 @@ def vne_section(w, pairs=25):
             "algorithmic_bytes_per_point": 16.0, "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
 
 
-def synthetic_batch(basis, count, atoms, grid_n, steps, seed):
-    """Seeded molecules with atom counts U{atoms}, cubic grids from grid_n, steps U(steps) bohr."""
-    from qmtools import synth
-    rng = numpy.random.default_rng(seed)
-    mols, grids = [], []
-    for i in range(count):
-        na = int(rng.integers(atoms[0], atoms[1] + 1))
-        n = int(rng.choice(grid_n))
-        st = float(rng.uniform(*steps))
-        mol = synth.synthetic_molecule(na, seed * 1000 + i, basis)
-        mols.append(mol)
-        grids.append(synth.centred_grid(mol, n, step_bohr=st, pad_ang=3.0))
-    return mols, grids
-
-
 def batched_section(basis, count, sweeps, seed=41):
     """Config 4 (reduced count): mixed batched fused launches vs the plain FP64 path (one molecule at a
     time through the drop-in API), for each Poisson method: Jacobi (hipGraph, `sweeps` sweeps, multipole
     boundary) and free-space FFT.  Same method and sweep count on both sides of each ratio."""
     import torch
     import qmtools
-    from qmtools import batch
-    mols, grids = synthetic_batch(basis, count, (5, 80), (64, 96, 128, 160, 192), (0.1, 0.15), seed)
+    from qmtools import batch, synth
+    mols, grids = synth.molecule_batch(count, (5, 80), (64, 96, 128, 160, 192), (0.1, 0.15), seed, basis)
     npts = batch.grid_points(grids)
     log(f"batched: {count} molecules, {npts} grid points, {sweeps} Jacobi sweeps")
     lane = batch.Lane(default=True)
@@ -324,8 +309,8 @@ def batched_section(basis, count, sweeps, seed=41):
 
 def served_section(basis, nreq, rate, sweeps, seed=77, grid_n=(64, 96, 128)):
     """Config 5: Poisson arrivals at `rate` req/s into the continuous-batching server (mixed path)."""
-    from qmtools import serve
-    mols, grids = synthetic_batch(basis, nreq, (5, 120), grid_n, (0.1, 0.15), seed)
+    from qmtools import serve, synth
+    mols, grids = synth.molecule_batch(nreq, (5, 120), grid_n, (0.1, 0.15), seed, basis)
     srv = serve.Server(nlanes=2, precision="mixed", max_batch=8, sweeps=sweeps, hbm_budget=96 << 30,
                        to_host=False).start()
     srv.submit(mols[0], grids[0]).wait(600)                        # warm-up

@@ -92,12 +92,24 @@ int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr,
 /* Sets the alpha-spin density matrix (nbf x nbf, row-major, host memory).  The
  * kernels use its lower triangle, mirrored, times 2 for the closed shell, as
  * the reference kernels do (kernel_density.cu:74-147, kernel_hartree.cu:836-838).
+ * The upload is ordered on `stream` (work queued earlier on it still sees the old
+ * matrix; work on other streams that uses the molecule is ordered by the caller) and
+ * waits for that stream only, never for the whole device.
  * Replaces the d_dm upload of Molecule.__init__ (qmtools.py:176). */
-int qm_mol_set_dm(qm_ctx* ctx, qm_mol* mol, const double* dm);
+int qm_mol_set_dm(qm_ctx* ctx, qm_mol* mol, void* stream, const double* dm);
 
-/* New nuclear positions (bohr).  Replaces Molecule.Translate's re-upload (qmtools.py:182-188). */
-int qm_mol_set_coords(qm_ctx* ctx, qm_mol* mol, const double* xyz_bohr);
+/* New nuclear positions (bohr), ordered on `stream` like qm_mol_set_dm.  The basis
+ * tables and the density matrix stay resident; the GTO potential's distribution
+ * table is rebuilt on its next use.
+ * Replaces Molecule.Translate's re-upload (qmtools.py:182-188). */
+int qm_mol_set_coords(qm_ctx* ctx, qm_mol* mol, void* stream, const double* xyz_bohr);
 int qm_mol_nbf(const qm_mol* mol);
+
+/* Resident-data counters of a molecule handle: counts[0] = density-matrix uploads,
+ * counts[1] = coordinate updates, counts[2] = builds of the GTO distribution table.
+ * No reference counterpart (the reference re-uploads per call); lets a caller check
+ * that a moved molecule reuses its resident basis tables and density matrix. */
+int qm_mol_stats(const qm_mol* mol, int64_t* counts);
 int qm_mol_destroy(qm_ctx* ctx, qm_mol* mol);
 
 /* ---------------------------------------------------------------- density */

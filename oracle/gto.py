@@ -152,14 +152,18 @@ def pair_integrals(system, sa, sb, pts):
     return out
 
 
-def potential(system, pts, nuclei=True):
-    """Total potential (a.u.) at points pts (npts, 3) bohr."""
+def potential(system, pts, nuclei=True, shells_a=None):
+    """Total potential (a.u.) at points pts (npts, 3) bohr.
+
+    shells_a restricts the outer shell index of the pair sum (sa >= sb) to an iterable of
+    shells, so that disjoint subsets can be summed by separate processes (with nuclei once).
+    """
     pts = numpy.asarray(pts, dtype=numpy.float64).reshape(-1, 3)
     Pt = 2.0 * lower_symmetric(system.dm)
     offs = _bf_offsets(system)
     V = numpy.zeros(pts.shape[0])
     ns = len(system.shells)
-    for sa in range(ns):
+    for sa in (range(ns) if shells_a is None else shells_a):
         La = system.shells[sa][1]
         for sb in range(sa + 1):
             Lb = system.shells[sb][1]

@@ -16,6 +16,7 @@ expansion of the charges rho placed at the nodes origin + i h, expanded about
 the box centre.  The initial guess then carries V_b on the outer layer.
 """
 import numpy
+from scipy import fft as sfft
 
 
 def _view(a, nx, ny, nz):
@@ -63,6 +64,39 @@ def jacobi(rho, nx, ny, nz, h, nsweeps, bvals=None, v0=None):
     for _ in range(int(nsweeps)):
         v = sweep(v, rho, nx, ny, nz, h, bvals)
     return v
+
+
+def jacobi_closed_form(rho, nx, ny, nz, h, nsweeps, bvals=None):
+    """The same m = nsweeps sweeps from 0 (boundary held at bvals) without iterating.
+
+    On the interior the sweep is V' = G V + c with G = (sum of the 6 neighbours) / 6
+    under zero Dirichlet conditions and c = (held boundary neighbours - rho / h) / 6.
+    DST-I diagonalises G, eigenvalue mu = (sum_axes 2 cos(pi k / (n_axis - 1))) / 6, so the
+    m-th iterate from V = 0 is V_m = sum_{j<m} G^j c = IDST(DST(c) (1 - mu^m) / (1 - mu)).
+    Costs a few transforms for any sweep count (2000 sweeps on 96^3 in seconds).
+    """
+    R = _view(rho, nx, ny, nz)
+    B = numpy.zeros_like(R) if bvals is None else numpy.where(
+        boundary_mask(nx, ny, nz).reshape(nz, ny, nx), _view(bvals, nx, ny, nz), 0.0)
+    s = numpy.zeros_like(B)
+    s[:, :, 1:] += B[:, :, :-1]
+    s[:, :, :-1] += B[:, :, 1:]
+    s[:, 1:, :] += B[:, :-1, :]
+    s[:, :-1, :] += B[:, 1:, :]
+    s[1:, :, :] += B[:-1, :, :]
+    s[:-1, :, :] += B[1:, :, :]
+    c = (s[1:-1, 1:-1, 1:-1] - R[1:-1, 1:-1, 1:-1] / h) / 6.0
+    mz, my, mx = c.shape
+    mu = numpy.zeros_like(c)
+    for ax, m in zip((2, 1, 0), (mx, my, mz)):
+        shape = [1, 1, 1]
+        shape[ax] = m
+        mu = mu + (2.0 * numpy.cos(numpy.pi * numpy.arange(1, m + 1) / (m + 1))).reshape(shape)
+    mu = mu / 6.0
+    vh = sfft.dstn(c, type=1) * (1.0 - mu ** int(nsweeps)) / (1.0 - mu)
+    out = B.copy()
+    out[1:-1, 1:-1, 1:-1] = sfft.idstn(vh, type=1)
+    return out.ravel()
 
 
 def hartree_iter(rho, nx, ny, nz, h, tolerance=1e-24, bvals=None, max_iterations=None):

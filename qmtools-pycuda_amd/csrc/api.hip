@@ -29,19 +29,18 @@ int qm_enter(qm_ctx* ctx) {
   return QM_OK;
 }
 
-int qm_ws(qm_ctx* ctx, int slot, size_t bytes, void** out) {
+int qm_ws(qm_ctx* ctx, hipStream_t s, int slot, size_t bytes, void** out) {
   Workspace& w = ctx->ws[slot];
   if (w.bytes < bytes) {
     if (w.ptr) {
-      // the old buffer may still be read by queued work
-      QM_CHECK_HIP(ctx, hipDeviceSynchronize());
-      QM_CHECK_HIP(ctx, hipFree(w.ptr));
+      // stream-ordered: work queued before this point on s still reads the old buffer
+      QM_CHECK_HIP(ctx, hipFreeAsync(w.ptr, s));
       w.ptr = nullptr;
       w.bytes = 0;
     }
     size_t want = std::max(bytes, (size_t)4096);
     want = (want + 4095) & ~(size_t)4095;
-    if (hipMalloc(&w.ptr, want) != hipSuccess) {
+    if (hipMallocAsync(&w.ptr, want, s) != hipSuccess) {
       (void)hipGetLastError();
       w.ptr = nullptr;
       return qm_fail(ctx, QM_ENOMEM, "workspace %d: cannot allocate %zu bytes", slot, want);
@@ -93,7 +92,8 @@ int qm_ctx_destroy(qm_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
   for (auto& w : ctx->ws)
-    if (w.ptr) (void)hipFree(w.ptr);
+    if (w.ptr) (void)hipFreeAsync(w.ptr, nullptr);
+  (void)hipStreamSynchronize(nullptr);
   if (ctx->host_scalars) (void)hipHostFree(ctx->host_scalars);
   qm_reclaim_graphs(ctx, true);
   qm_fft_cache_free(ctx->fft);
@@ -246,7 +246,7 @@ int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr,
   return QM_OK;
 }
 
-int qm_mol_set_dm(qm_ctx* ctx, qm_mol* mol, const double* dm) {
+int qm_mol_set_dm(qm_ctx* ctx, qm_mol* mol, void* stream, const double* dm) {
   int rc = qm_enter(ctx);
   if (rc) return rc;
   QM_REQUIRE(ctx, mol && dm, QM_EINVAL, "null argument");
@@ -260,14 +260,15 @@ int qm_mol_set_dm(qm_ctx* ctx, qm_mol* mol, const double* dm) {
       mol->h_P[(size_t)q * n + p] = 2.0 * v;
     }
   }
-  rc = qm_density_upload_P(ctx, mol);
+  rc = qm_density_upload_P(ctx, mol, (hipStream_t)stream);
   if (rc) return rc;
   mol->has_dm = true;
+  mol->n_p_uploads++;
   mol->gto_dirty = true;
   return QM_OK;
 }
 
-int qm_mol_set_coords(qm_ctx* ctx, qm_mol* mol, const double* xyz_bohr) {
+int qm_mol_set_coords(qm_ctx* ctx, qm_mol* mol, void* stream, const double* xyz_bohr) {
   int rc = qm_enter(ctx);
   if (rc) return rc;
   QM_REQUIRE(ctx, mol && xyz_bohr, QM_EINVAL, "null argument");
@@ -279,10 +280,20 @@ int qm_mol_set_coords(qm_ctx* ctx, qm_mol* mol, const double* xyz_bohr) {
     at[a] = make_double4(xyz_bohr[3 * a], xyz_bohr[3 * a + 1], xyz_bohr[3 * a + 2], mol->Z[a]);
   }
   mol->xyz.assign(xyz_bohr, xyz_bohr + 3 * mol->natoms);
-  // queued kernels may still read the old table
-  QM_CHECK_HIP(ctx, hipDeviceSynchronize());
-  QM_CHECK_HIP(ctx, hipMemcpy(mol->atoms, at.data(), sizeof(double4) * mol->natoms, hipMemcpyHostToDevice));
+  // ordered on the stream: kernels queued earlier on it read the old table, later ones the new
+  hipStream_t s = (hipStream_t)stream;
+  QM_CHECK_HIP(ctx, hipMemcpyAsync(mol->atoms, at.data(), sizeof(double4) * mol->natoms, hipMemcpyHostToDevice, s));
+  QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // the host table is pageable and goes out of scope
   mol->gto_dirty = true;
+  mol->n_coord_updates++;
+  return QM_OK;
+}
+
+int qm_mol_stats(const qm_mol* mol, int64_t* counts) {
+  if (!mol || !counts) return QM_EINVAL;
+  counts[0] = mol->n_p_uploads;
+  counts[1] = mol->n_coord_updates;
+  counts[2] = mol->n_gto_builds;
   return QM_OK;
 }
 

@@ -1481,7 +1481,7 @@ constexpr size_t kPhiBudget = (size_t)4 << 30;   // bytes of basis values per ch
 
 }  // namespace
 
-int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol) {
+int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol, hipStream_t s) {
   const int n = mol->nbf, np = mol->nbf_pad;
   std::vector<double> P((size_t)np * np, 0.0);
   double pmax = 0.0;
@@ -1503,22 +1503,21 @@ int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol) {
       return qm_fail(ctx, QM_ENOMEM, "density matrix: device allocation failed");
     }
   }
-  // queued kernels may still read the previous matrix
-  QM_CHECK_HIP(ctx, hipDeviceSynchronize());
-  QM_CHECK_HIP(ctx, hipMemcpy(mol->P64, P.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  // ordered on s: kernels queued earlier on s still read the previous matrix
+  QM_CHECK_HIP(ctx, hipMemcpyAsync(mol->P64, P.data(), sizeof(double) * tot, hipMemcpyHostToDevice, s));
   // M = D + 2U (upper triangle of P, off-diagonal doubled): phi^T P phi = phi^T M phi
   std::vector<double> Mh((size_t)np * np, 0.0);
   for (int p = 0; p < np; ++p)
     for (int q = p; q < np; ++q) Mh[(size_t)p * np + q] = (p == q ? 1.0 : 2.0) * P[(size_t)p * np + q];
-  QM_CHECK_HIP(ctx, hipMemcpy(mol->M64, Mh.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  QM_CHECK_HIP(ctx, hipMemcpyAsync(mol->M64, Mh.data(), sizeof(double) * tot, hipMemcpyHostToDevice, s));
   const unsigned blocks = (unsigned)((tot + 255) / 256);
-  hipLaunchKernelGGL(k_split_P<_Float16>, dim3(blocks), dim3(256), 0, 0, (const double*)mol->M64, np,
+  hipLaunchKernelGGL(k_split_P<_Float16>, dim3(blocks), dim3(256), 0, s, (const double*)mol->M64, np,
                      (float)mol->pfac, (_Float16*)mol->P16, (_Float16*)mol->P16 + tot);
   QM_CHECK_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_split_P<__bf16>, dim3(blocks), dim3(256), 0, 0, (const double*)mol->M64, np,
+  hipLaunchKernelGGL(k_split_P<__bf16>, dim3(blocks), dim3(256), 0, s, (const double*)mol->M64, np,
                      (float)mol->pfac, (__bf16*)mol->Pb16, (__bf16*)mol->Pb16 + tot);
   QM_CHECK_LAUNCH(ctx);
-  QM_CHECK_HIP(ctx, hipDeviceSynchronize());
+  QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // the host matrices are pageable and go out of scope
   return QM_OK;
 }
 
@@ -1582,17 +1581,17 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
   nvox = std::min<int64_t>(nvox, ((int64_t)1 << 30) / nsub3);
   const int64_t ldmax = round_up(nvox * nsub3, kTileM);
   void* phi = nullptr;
-  int rc = qm_ws(ctx, WS_PHI, sizeof(double) * (size_t)np * ldmax, &phi);
+  int rc = qm_ws(ctx, s, WS_PHI, sizeof(double) * (size_t)np * ldmax, &phi);
   if (rc) return rc;
   double* rho = nullptr;
   if (nsub3 > 1) {
-    rc = qm_ws(ctx, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
+    rc = qm_ws(ctx, s, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
     if (rc) return rc;
   }
   const bool tiled = ctx->f64_variant >= 2;   // 2 tiles, 3 rows, 4 tiles fed by LDS DMA
   const int ncb = ctx->f64_variant == 3 ? np / kRN : (np + kTT - 1) / kTT;
   double* part = nullptr;
-  if (tiled && (rc = qm_ws(ctx, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
+  if (tiled && (rc = qm_ws(ctx, s, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
   const GeomView gv = geom_view(g);
   const MolView mv = mol->view();
   const double h3 = g->step * g->step * g->step;
@@ -1706,14 +1705,14 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
   nvox = std::min<int64_t>(nvox, ((int64_t)1 << 30) / nsub3);
   const int64_t ldmax = round_up(nvox * nsub3, kTileM);
   void* phi = nullptr;
-  int rc = qm_ws(ctx, WS_PHI, sizeof(float) * (size_t)np * ldmax, &phi);
+  int rc = qm_ws(ctx, s, WS_PHI, sizeof(float) * (size_t)np * ldmax, &phi);
   if (rc) return rc;
   float* pscale = nullptr;
-  rc = qm_ws(ctx, WS_PHI2, sizeof(float) * ldmax, (void**)&pscale);
+  rc = qm_ws(ctx, s, WS_PHI2, sizeof(float) * ldmax, (void**)&pscale);
   if (rc) return rc;
   double* rho = nullptr;
   if (nsub3 > 1) {
-    rc = qm_ws(ctx, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
+    rc = qm_ws(ctx, s, WS_RHO, sizeof(double) * ldmax, (void**)&rho);
     if (rc) return rc;
   }
   const T16* Phi = (const T16*)(std::is_same<T16, _Float16>::value ? mol->P16 : mol->Pb16);
@@ -1723,7 +1722,7 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
   const float S = split_phi_scale(mol);
   const int ncb = (np + kTT - 1) / kTT;
   double* part = nullptr;
-  if (tiled && (rc = qm_ws(ctx, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
+  if (tiled && (rc = qm_ws(ctx, s, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
   const GeomView gv = geom_view(g);
   const MolView mv = mol->view();
   const double h3 = g->step * g->step * g->step / mol->pfac;
@@ -1812,7 +1811,7 @@ int qm_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g, int sub
   if (rc) return rc;
   if (normalize || raw_total) {
     double* dtot = nullptr;
-    rc = qm_ws(ctx, WS_SCALAR, 256, (void**)&dtot);
+    rc = qm_ws(ctx, s, WS_SCALAR, 256, (void**)&dtot);
     if (rc) return rc;
     rc = qm_reduce_dev(ctx, s, QM_RED_SUM, out_dtype, out, nullptr, npoints(g), dtot);
     if (rc) return rc;
@@ -1899,7 +1898,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   const int64_t plane = (int64_t)g->nx * g->ny;
   slab.nz = (int)std::max<int64_t>(1, std::min<int64_t>(g->nz, ((int64_t)1 << 21) / std::max<int64_t>(plane, 1)));
   void* out = nullptr;
-  if ((rc = qm_ws(ctx, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
   std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 64, 128} : std::vector<int>{0, 1, 2, 3, 4};
   int& knob = split ? ctx->split_tm : ctx->f64_variant;
@@ -2252,10 +2251,10 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
     max_part = std::max(max_part, q);
   }
   void *phi = nullptr, *ps = nullptr, *dtab = nullptr, *pt = nullptr;
-  if ((rc = qm_ws(ctx, WS_PHI, max_bytes, &phi))) return rc;
-  if ((rc = qm_ws(ctx, WS_PHI2, std::max<size_t>(max_ps, 16), &ps))) return rc;
-  if (tiled && (rc = qm_ws(ctx, WS_PART, max_part, &pt))) return rc;
-  if ((rc = qm_ws(ctx, WS_BATCH, sizeof(BChunk) * table.size(), &dtab))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_PHI, max_bytes, &phi))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_PHI2, std::max<size_t>(max_ps, 16), &ps))) return rc;
+  if (tiled && (rc = qm_ws(ctx, s, WS_PART, max_part, &pt))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_BATCH, sizeof(BChunk) * table.size(), &dtab))) return rc;
   for (BChunk& c : table) {
     c.phi = (char*)phi + (size_t)(uintptr_t)c.phi;
     c.pscale = (float*)((char*)ps + (size_t)(uintptr_t)c.pscale);
@@ -2320,7 +2319,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   }
   if (normalize) {
     double* tot = nullptr;
-    if ((rc = qm_ws(ctx, WS_BATCH2, sizeof(double) * (size_t)nmol, (void**)&tot))) return rc;
+    if ((rc = qm_ws(ctx, s, WS_BATCH2, sizeof(double) * (size_t)nmol, (void**)&tot))) return rc;
     for (int i = 0; i < nmol; ++i) {
       if ((rc = qm_reduce_dev(ctx, s, QM_RED_SUM, out_dtype, outs[i], nullptr, npoints(&geoms[i]), tot + i))) return rc;
       if ((rc = qm_scale_dev(ctx, s, out_dtype, outs[i], npoints(&geoms[i]), nullptr, mols[i]->qtot, tot + i)))

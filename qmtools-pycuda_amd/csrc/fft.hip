@@ -592,7 +592,7 @@ int hartree_fft(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, const
   const int m[3] = {fs ? 2 * n[0] : n[0], fs ? 2 * n[1] : n[1], fs ? 2 * n[2] : n[2]};
   const int64_t M = (int64_t)m[0] * m[1] * m[2];
   Cx<R>* work = nullptr;
-  int rc = qm_ws(ctx, WS_FFT_A, sizeof(Cx<R>) * (size_t)M, (void**)&work);
+  int rc = qm_ws(ctx, s, WS_FFT_A, sizeof(Cx<R>) * (size_t)M, (void**)&work);
   if (rc) return rc;
   const R* gh = nullptr;
   if (fs) {

@@ -301,7 +301,7 @@ struct GtoCache {
   double* boys = nullptr;
 };
 
-int build_distributions(qm_ctx* ctx, qm_mol* mol, GtoCache& c) {
+int build_distributions(qm_ctx* ctx, qm_mol* mol, hipStream_t s, GtoCache& c) {
   const int nsh = mol->nshell, nbf = mol->nbf;
   // Cartesian term lists per (L, m index)
   std::vector<CartTerm> terms[3][5];
@@ -400,17 +400,20 @@ int build_distributions(qm_ctx* ctx, qm_mol* mol, GtoCache& c) {
     c.n[L] = (int)c.lists[L].size();
     tot += c.lists[L].size();
   }
+  // stream-ordered: kernels queued earlier on s still read the previous table
   if (c.dev) {
-    (void)hipFree(c.dev);
+    QM_CHECK_HIP(ctx, hipFreeAsync(c.dev, s));
     c.dev = nullptr;
   }
   if (tot) {
-    QM_CHECK_HIP(ctx, hipMalloc(&c.dev, sizeof(HDist) * tot));
+    QM_CHECK_HIP(ctx, hipMallocAsync((void**)&c.dev, sizeof(HDist) * tot, s));
     for (int L = 0; L <= kLmax; ++L)
       if (c.n[L])
-        QM_CHECK_HIP(ctx, hipMemcpy(c.dev + c.off[L], c.lists[L].data(), sizeof(HDist) * c.n[L],
-                                    hipMemcpyHostToDevice));
+        QM_CHECK_HIP(ctx, hipMemcpyAsync(c.dev + c.off[L], c.lists[L].data(), sizeof(HDist) * c.n[L],
+                                         hipMemcpyHostToDevice, s));
+    QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // the host lists are rebuilt in place next time
   }
+  mol->n_gto_builds++;
   if (!c.boys) {
     const std::vector<double> tab = make_boys_table();
     QM_CHECK_HIP(ctx, hipMalloc(&c.boys, sizeof(double) * tab.size()));
@@ -419,13 +422,11 @@ int build_distributions(qm_ctx* ctx, qm_mol* mol, GtoCache& c) {
   return QM_OK;
 }
 
-int get_cache(qm_ctx* ctx, qm_mol* mol, GtoCache** out) {
+int get_cache(qm_ctx* ctx, qm_mol* mol, hipStream_t s, GtoCache** out) {
   if (!mol->gto) mol->gto = new GtoCache();
   GtoCache* c = (GtoCache*)mol->gto;
   if (mol->gto_dirty) {
-    // queued kernels may still read the previous tables
-    QM_CHECK_HIP(ctx, hipDeviceSynchronize());
-    int rc = build_distributions(ctx, mol, *c);
+    int rc = build_distributions(ctx, mol, s, *c);
     if (rc) return rc;
     mol->gto_dirty = false;
   }
@@ -437,7 +438,7 @@ template <typename TO>
 int gto_eval(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const double* pts, const GeomView& gv, int64_t npts,
              TO* out) {
   GtoCache* c = nullptr;
-  int rc = get_cache(ctx, mol, &c);
+  int rc = get_cache(ctx, mol, s, &c);
   if (rc) return rc;
   DistLists dl;
   int total = 0;
@@ -451,7 +452,7 @@ int gto_eval(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const double* pts, const G
   int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(256, 2048 / std::max<int64_t>(pblocks, 1)));
   nchunk = std::max(1, std::min(nchunk, std::max(1, total / 64)));
   double* partial = nullptr;
-  if ((rc = qm_ws(ctx, WS_GTO, sizeof(double) * (size_t)nchunk * npts, (void**)&partial))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_GTO, sizeof(double) * (size_t)nchunk * npts, (void**)&partial))) return rc;
   QM_REQUIRE(ctx, pblocks <= 0x7fffffff, QM_ELIMIT, "gto: too many points");
   {
     ProfScope ps(ctx, s, QM_K_GTO);
@@ -470,7 +471,10 @@ int gto_eval(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const double* pts, const G
 void qm_gto_cache_free(void* p) {
   GtoCache* c = (GtoCache*)p;
   if (!c) return;
-  if (c->dev) (void)hipFree(c->dev);
+  if (c->dev) {
+    (void)hipFreeAsync(c->dev, nullptr);
+    (void)hipStreamSynchronize(nullptr);
+  }
   if (c->boys) (void)hipFree(c->boys);
   delete c;
 }

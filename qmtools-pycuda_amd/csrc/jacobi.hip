@@ -475,7 +475,7 @@ inline void split_block(int64_t n, int64_t& P, int64_t& S) {
 
 int host_delta(qm_ctx* ctx, hipStream_t s, int dtype, const void* a, const void* b, int64_t n, double* out) {
   double* d = nullptr;
-  int rc = qm_ws(ctx, WS_SCALAR, 256, (void**)&d);
+  int rc = qm_ws(ctx, s, WS_SCALAR, 256, (void**)&d);
   if (rc) return rc;
   rc = qm_reduce_dev(ctx, s, QM_RED_ABSDIFF, dtype, a, b, n, d + 1);
   if (rc) return rc;
@@ -607,9 +607,9 @@ int jacobi_fp32(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const double* rho,
   const int64_t n = npoints(g);
   float *r = nullptr, *e = nullptr, *t = nullptr;
   int rc;
-  if ((rc = qm_ws(ctx, WS_JAC_A, sizeof(float) * n, (void**)&r))) return rc;
-  if ((rc = qm_ws(ctx, WS_JAC_B, sizeof(float) * n, (void**)&e))) return rc;
-  if ((rc = qm_ws(ctx, WS_JAC_C, sizeof(float) * n, (void**)&t))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_JAC_A, sizeof(float) * n, (void**)&r))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_JAC_B, sizeof(float) * n, (void**)&e))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_JAC_C, sizeof(float) * n, (void**)&t))) return rc;
   const unsigned upd_blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
   const bool checks = tol >= 0;
   // one residual block of m sweeps: residual, m FP32 sweeps, correction (kernels only).
@@ -700,8 +700,8 @@ int qm_multipole_boundary(qm_ctx* ctx, void* stream, const qm_geom* g, int dtype
   const int64_t seg = (n + nb - 1) / nb;
   double* part = nullptr;
   double* mom = nullptr;
-  if ((rc = qm_ws(ctx, WS_RED, sizeof(double) * 10 * kMomBlocks, (void**)&part))) return rc;
-  if ((rc = qm_ws(ctx, WS_SCALAR, 256, (void**)&mom))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_RED, sizeof(double) * 10 * kMomBlocks, (void**)&part))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_SCALAR, 256, (void**)&mom))) return rc;
   mom += 8;
   const GeomView gv = geom_view(g);
   if (dtype == QM_F64)
@@ -970,7 +970,7 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
   hipStream_t s = (hipStream_t)stream;
   if (!f64) {
     float* w = nullptr;
-    if ((rc = qm_ws(ctx, WS_JAC_A, sizeof(float) * 3 * f32_elems, (void**)&w))) return rc;
+    if ((rc = qm_ws(ctx, s, WS_JAC_A, sizeof(float) * 3 * f32_elems, (void**)&w))) return rc;
     size_t off = 0;
     for (int i = 0; i < n; ++i) {
       const size_t N = (size_t)npoints(&geoms[i]);
@@ -983,7 +983,7 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
     }
   }
   void* dtab = nullptr;
-  if ((rc = qm_ws(ctx, WS_BATCH, sizeof(JItem) * items.size(), &dtab))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_BATCH, sizeof(JItem) * items.size(), &dtab))) return rc;
   QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, items.data(), sizeof(JItem) * items.size(), hipMemcpyHostToDevice, s));
   QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // pageable source
   const JItem* dt = (const JItem*)dtab;

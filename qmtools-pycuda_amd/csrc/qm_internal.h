@@ -127,6 +127,9 @@ struct qm_mol {
   bool gto_dirty = true;            // set when P or the coordinates change
   double pfac = 1.0;                // power-of-two scale applied to P before the 16-bit split
   double qtot = 0.0;
+  int64_t n_p_uploads = 0;          // density-matrix uploads (qm_mol_set_dm)
+  int64_t n_coord_updates = 0;      // coordinate updates (qm_mol_set_coords)
+  int64_t n_gto_builds = 0;         // builds of the Hermite distribution tables
   MolView view() const {
     MolView v;
     v.atoms = atoms; v.shells = shells; v.prims = prims;
@@ -152,8 +155,10 @@ int qm_fail(qm_ctx* ctx, int code, const char* fmt, ...);
     if (!(cond)) return qm_fail((ctx), (code), __VA_ARGS__); \
   } while (0)
 
-// grow-only device workspace
-int qm_ws(qm_ctx* ctx, int slot, size_t bytes, void** out);
+// Grow-only device workspace from the stream-ordered allocator.  Growth frees the old
+// buffer and allocates the new one in the order of stream s, so work queued earlier on s
+// (or ordered before s by the caller) still reads the old buffer; no device-wide wait.
+int qm_ws(qm_ctx* ctx, hipStream_t s, int slot, size_t bytes, void** out);
 int qm_enter(qm_ctx* ctx);   // hipSetDevice + clear error
 
 inline int64_t npoints(const qm_geom* g) { return (int64_t)g->nx * g->ny * g->nz; }
@@ -166,7 +171,7 @@ int qm_reduce_dev(qm_ctx* ctx, hipStream_t s, int op, int dtype, const void* a, 
 int qm_scale_dev(qm_ctx* ctx, hipStream_t s, int dtype, void* a, int64_t n, const double* dev_num,
                  double num_const, const double* dev_den);
 // density-matrix device forms (density.hip)
-int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol);
+int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol, hipStream_t s);
 void qm_fft_cache_free(FftPlanCache* c);
 void qm_gto_cache_free(void* c);
 void qm_reclaim_graphs(qm_ctx* ctx, bool wait);

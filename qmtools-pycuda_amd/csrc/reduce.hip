@@ -126,7 +126,7 @@ int qm_reduce_dev(qm_ctx* ctx, hipStream_t s, int op, int dtype, const void* a, 
   if (nb > kRedMaxBlocks) nb = kRedMaxBlocks;
   int64_t seg = (n + nb - 1) / nb;
   void* part = nullptr;
-  int rc = qm_ws(ctx, WS_RED, sizeof(double) * kRedMaxBlocks, &part);
+  int rc = qm_ws(ctx, s, WS_RED, sizeof(double) * kRedMaxBlocks, &part);
   if (rc) return rc;
   double* p = (double*)part;
   if (dtype == QM_F64) {
@@ -171,7 +171,7 @@ int qm_reduce(qm_ctx* ctx, void* stream, int op, int dtype, const void* a, const
   QM_REQUIRE(ctx, result != nullptr, QM_EINVAL, "reduce: null result");
   hipStream_t s = (hipStream_t)stream;
   void* dres = nullptr;
-  rc = qm_ws(ctx, WS_SCALAR, 256, &dres);
+  rc = qm_ws(ctx, s, WS_SCALAR, 256, &dres);
   if (rc) return rc;
   rc = qm_reduce_dev(ctx, s, op, dtype, a, b, n, (double*)dres);
   if (rc) return rc;

@@ -319,7 +319,7 @@ int vne_diffusion(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, dou
   QM_REQUIRE(ctx, nb < (1 << 30), QM_ELIMIT, "vne: grid too large");
   void* w = nullptr;
   int rc;
-  if ((rc = qm_ws(ctx, WS_VNE, sizeof(double) * (size_t)(nb + 8), &w))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_VNE, sizeof(double) * (size_t)(nb + 8), &w))) return rc;
   VneState* st = (VneState*)w;
   double* part = (double*)w + 8;
   QM_CHECK_HIP(ctx, hipMemsetAsync(v, 0, sizeof(T) * n, s));

@@ -55,9 +55,10 @@ SIGNATURES = {
     "qm_ctx_destroy": [_vp],
     "qm_last_error": [_vp],
     "qm_mol_create": [_vp, _i, _pi, _pd, _i, _pi, _pi, _pi, _pi, _i, _pd, _pd, _c.POINTER(_vp)],
-    "qm_mol_set_dm": [_vp, _vp, _pd],
-    "qm_mol_set_coords": [_vp, _vp, _pd],
+    "qm_mol_set_dm": [_vp, _vp, _vp, _pd],
+    "qm_mol_set_coords": [_vp, _vp, _vp, _pd],
     "qm_mol_nbf": [_vp],
+    "qm_mol_stats": [_vp, _c.POINTER(_i64)],
     "qm_mol_destroy": [_vp, _vp],
     "qm_density": [_vp, _vp, _vp, _pg, _i, _i, _i, _vp, _i, _pd],
     "qm_basis_points": [_vp, _vp, _vp, _vp, _i64, _vp],

@@ -15,12 +15,11 @@ accepts only the pickle globals NumPy arrays need and refuses every other one.
 from __future__ import annotations
 
 import glob
-import io
 import os
-import pickle
 
 import numpy
 
+from . import _safeload
 from .grid import Grid
 from .molecule import Molecule
 
@@ -55,25 +54,9 @@ def read_entry(path):
     return out
 
 
-_NUMPY_GLOBALS = {
-    ("numpy.core.multiarray", "_reconstruct"), ("numpy._core.multiarray", "_reconstruct"),
-    ("numpy.core.multiarray", "scalar"), ("numpy._core.multiarray", "scalar"),
-    ("numpy.core.numeric", "_frombuffer"), ("numpy._core.numeric", "_frombuffer"),
-    ("numpy", "ndarray"), ("numpy", "dtype"),
-}
-
-
-class _ArrayOnlyUnpickler(pickle.Unpickler):
-    def find_class(self, module, name):
-        if (module, name) in _NUMPY_GLOBALS:
-            return super().find_class(module, name)
-        raise pickle.UnpicklingError(f"refused pickle global {module}.{name}: only NumPy arrays are accepted")
-
-
 def read_legacy_pickle(path):
     """Fields of a reference-layout `molecule.<CID>.pickle` (dicts, lists, numbers, strings, arrays only)."""
-    with open(path, "rb") as fh:
-        d = _ArrayOnlyUnpickler(io.BytesIO(fh.read())).load()
+    d = _safeload.load(path)
     if not isinstance(d, dict):
         raise ValueError(f"{path}: not a molecule cache entry")
     return {k: d.get(k) for k in KEYS}

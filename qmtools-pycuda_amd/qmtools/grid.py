@@ -24,7 +24,7 @@ import struct
 
 import numpy
 
-from . import _device
+from . import _device, _safeload
 
 ANG2BOR = 1.8897259886
 _F3 = numpy.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8")])
@@ -164,10 +164,12 @@ class Grid:
     def Load(molecule, filename):
         """Restore a grid written by `Save` (reference `qmtools.py:303-314`).
 
-        Like the reference this unpickles the file: only load files this code wrote.
+        The file is read with an unpickler that rebuilds plain data and NumPy arrays
+        only and refuses any other pickle global, so loading runs nothing from the file.
         """
-        with open(filename, "rb") as fh:
-            d = pickle.load(fh)
+        d = _safeload.load(filename)
+        if not isinstance(d, dict) or not {"shape", "origin", "step", "lattice", "qube"} <= set(d):
+            raise ValueError(f"{filename}: not a grid written by Grid.Save")
         q = numpy.asarray(d["qube"])
         grid = Grid(d["shape"], 0.1, dtype=q.dtype if q.dtype in (numpy.float32, numpy.float64) else numpy.float64)
         o = d["origin"]

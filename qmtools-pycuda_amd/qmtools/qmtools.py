@@ -55,11 +55,18 @@ class _DeviceMol:
 
     def set_dm(self, dm):
         d = numpy.ascontiguousarray(dm, dtype=numpy.float64)
-        nat.check(nat.library().qm_mol_set_dm(self._ctx, self.handle, nat.f64p(d)), self._ctx)
+        nat.check(nat.library().qm_mol_set_dm(self._ctx, self.handle, nat.stream(), nat.f64p(d)), self._ctx)
 
     def update_coords(self, coords):
+        """New nuclear positions, ordered on the current stream; P and the basis tables stay resident."""
         xyz = numpy.ascontiguousarray(coords, dtype=numpy.float64)
-        nat.check(nat.library().qm_mol_set_coords(self._ctx, self.handle, nat.f64p(xyz)), self._ctx)
+        nat.check(nat.library().qm_mol_set_coords(self._ctx, self.handle, nat.stream(), nat.f64p(xyz)), self._ctx)
+
+    def stats(self):
+        """{'p_uploads', 'coord_updates', 'gto_builds'} counters of the resident tables."""
+        c = (ctypes.c_int64 * 3)()
+        nat.check(nat.library().qm_mol_stats(self.handle, c), self._ctx)
+        return {"p_uploads": c[0], "coord_updates": c[1], "gto_builds": c[2]}
 
     def __del__(self):
         try:

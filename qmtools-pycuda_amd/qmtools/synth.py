@@ -141,3 +141,45 @@ def uniform_points(grid: Grid, npoints: int, seed: int):
     lo = numpy.array(o)
     ext = numpy.array([nx, ny, nz], dtype=numpy.float64) * h
     return lo + rng.uniform(0.0, 1.0, size=(npoints, 3)) * ext
+
+
+def molecule_12_grid_96(basis: BasisSet | None = None):
+    """Seeded 12-atom molecule (about 130 basis functions) on a 96^3 grid, 3 Angstrom padding:
+    the CPU-sized density + 2000-sweep Jacobi workload."""
+    mol = synthetic_molecule(12, 1, basis)
+    return mol, centred_grid(mol, 96, step_bohr=0.1, pad_ang=3.0)
+
+
+def molecule_60_grid_256(basis: BasisSet | None = None, npoints: int = 10000):
+    """Seeded 60-atom molecule (606 basis functions) on a 256^3 grid, 3 Angstrom padding, and
+    `npoints` uniformly drawn points for the GTO potential: the full-size density + Hartree workload."""
+    mol = synthetic_molecule(60, 2, basis)
+    grid = centred_grid(mol, 256, step_bohr=0.1, pad_ang=3.0)
+    return mol, grid, uniform_points(grid, npoints, 3)
+
+
+def dm_probe(dm):
+    """A fingerprint of a density matrix: dm times a fixed vector (compares to ~1e-13 across BLAS
+    builds, where the QR of the generator may round differently in the last bits)."""
+    n = dm.shape[0]
+    v = numpy.cos(0.37 * numpy.arange(n)) + 0.5
+    return dm @ v
+
+
+def molecule_batch(count: int, atoms=(5, 80), grid_n=(64, 96, 128, 160, 192), steps=(0.1, 0.15), seed: int = 41,
+                   basis: BasisSet | None = None):
+    """`count` seeded molecules with atom counts U{atoms}, cubic grids drawn from grid_n and grid steps
+    U(steps) bohr (grown where the molecule plus 3 Angstrom of padding does not fit): the batched and
+    served workloads (defaults: 5-80 atoms, 64^3-192^3 grids at 0.1-0.15 bohr)."""
+    if basis is None:
+        basis = BasisSet(default_basis_path())
+    rng = numpy.random.default_rng(seed)
+    mols, grids = [], []
+    for i in range(count):
+        na = int(rng.integers(atoms[0], atoms[1] + 1))
+        n = int(rng.choice(grid_n))
+        st = float(rng.uniform(*steps))
+        mol = synthetic_molecule(na, seed * 1000 + i, basis)
+        mols.append(mol)
+        grids.append(centred_grid(mol, n, step_bohr=st, pad_ang=3.0))
+    return mols, grids

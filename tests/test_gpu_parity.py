@@ -89,7 +89,7 @@ def test_density_fp64_mol29766(qm, mol29766):
     assert q.raw_total == pytest.approx(raw, rel=1e-10)
 
 
-@pytest.mark.parametrize("prec,tol", [("f16", 2e-6), ("bf16", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("f16", 1e-6), ("bf16", 2e-5)])
 def test_density_mixed(qm, small, prec, tol):
     from oracle import density as od
     mol, grid, sysm = small
@@ -117,10 +117,10 @@ def test_density_block_alignment(qm, obasis, natoms, seed):
     q = qm.QMTools.Compute_density(grid, mol)
     assert _rel(q.qube, want) <= 1e-10
     q = qm.QMTools.Compute_density(grid, mol, precision="mixed")
-    assert _rel(q.qube, want) <= 2e-6
+    assert _rel(q.qube, want) <= 1e-6
 
 
-@pytest.mark.parametrize("prec,tol", [("f16", 2e-6), ("bf16", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("f16", 1e-6), ("bf16", 2e-5)])
 @pytest.mark.parametrize("subgrid", [1, 2])
 @pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_density_mixed_tiled(qm, small, prec, tol, subgrid, variant):
@@ -353,8 +353,8 @@ def test_fft_mixed_radix_lengths(qm, small, shape):
     nx, ny, nz, h, o = grid.geometry()
     for mode in ("periodic", "freespace"):
         want = off.hartree_fft(grid.qube.ravel(), mol.types, mol.coords, nx, ny, nz, h, o, mode=mode)
-        # FP64 to the oracle's precision; FP32 (complex64 passes, their own LDS layout) to its own
-        for precision, tol in (("fp64", 1e-10), ("mixed", 1e-5)):
+        # FP64 to the oracle's precision; FP32 (complex64 passes) to the mixed path's 1e-6
+        for precision, tol in (("fp64", 1e-10), ("mixed", 1e-6)):
             _say("fft radix", shape, mode, precision)
             v = qm.QMTools.Compute_hartree_fft(grid, mol, mode=mode, precision=precision)
             assert _rel(v, want) <= tol, (mode, precision)
@@ -370,7 +370,7 @@ def test_fft_fp32(qm, small):
     want = off.hartree_fft(q.qube.ravel(), mol.types, mol.coords, nx, ny, nz, h, o, mode="freespace")
     err = _rel(v, want)
     print(f"FP32 free-space FFT normwise error {err:.3e}")
-    assert err <= 1e-5
+    assert err <= 1e-6
 
 
 # ---------------------------------------------------------------- GTO
@@ -484,3 +484,49 @@ def test_seed_shared_nodes_deterministic(qm):
     for _ in range(3):
         s = qm.QMTools.Compute_qseed(grid, mol)
         numpy.testing.assert_allclose(s.qube.ravel(), want, rtol=1e-14, atol=1e-15)
+
+
+# ---------------------------------------------------------------- Translate with resident tables
+def test_translate_reuses_resident_tables(qm, obasis):
+    """Molecule.Translate (reference qmtools.py:182-188) moves the nuclei on the device: density, GTO
+    potential at points, free-space FFT potential and the batched density all follow the moved
+    nuclei (oracle at the new coordinates, 1e-10), the GTO table is rebuilt once, and neither the
+    density matrix nor the basis tables are uploaded again."""
+    from oracle import density as od
+    from oracle import fft as off
+    from oracle import gto as og
+    from qmtools import batch, synth
+    from qmtools.qmtools import device_molecule
+    basis = qm.BasisSet(BASIS)
+    mol = synth.synthetic_molecule(7, 33, basis)
+    grid = synth.centred_grid(mol, 32, step_bohr=0.35, pad_ang=2.5)
+    pts = synth.uniform_points(grid, 40, 8)
+    nx, ny, nz, h, o = grid.geometry()
+    _say("translate: before")
+    qm.QMTools.Compute_density(grid, mol)
+    qm.QMTools.Compute_hartree_points(mol, pts)
+    dev = device_molecule(mol)
+    before = dev.stats()
+    assert before == {"p_uploads": 1, "coord_updates": 0, "gto_builds": 1}
+    shift = (0.31, -0.47, 0.22)    # bohr, as the reference's Translate
+    moved = mol.coords + numpy.array(shift)
+    mol.Translate(*shift)
+    numpy.testing.assert_array_equal(mol.coords, moved)
+    assert device_molecule(mol) is dev
+    sysm = _system(mol, obasis)
+    _say("translate: density, GTO, FFT after the move")
+    q = qm.QMTools.Compute_density(grid, mol)
+    want, raw = od.density_grid(sysm, nx, ny, nz, h, o)
+    assert _rel(q.qube, want) <= 1e-10
+    v = qm.QMTools.Compute_hartree_points(mol, pts)
+    assert _rel(v, og.potential(sysm, pts)) <= 1e-10
+    vf = qm.QMTools.Compute_hartree_fft(q, mol, mode="freespace")
+    vfw = off.hartree_fft(q.qube.ravel(), mol.types, moved, nx, ny, nz, h, o, mode="freespace")
+    assert _rel(vf, vfw) <= 1e-10
+    qb = batch.density_batch([mol], [grid], "fp64")[0]
+    numpy.testing.assert_array_equal(qb.cpu().numpy(), q.qube.ravel())
+    after = dev.stats()
+    assert after == {"p_uploads": 1, "coord_updates": 1, "gto_builds": 2}
+    # a second GTO call reuses the rebuilt table
+    qm.QMTools.Compute_hartree_points(mol, pts)
+    assert dev.stats()["gto_builds"] == 2

@@ -1,7 +1,9 @@
 """GPU parity of the diffusion pseudo-V_ne relaxation (Compute_VNe) against the FP64 oracle.
 
 FP64: field within 1e-12 (normwise relative), same pair count and stop reason.
-FP32 storage: 21 fixed pairs within 2e-6.  Inputs are printed before each GPU step.
+FP32 storage (an optional variant; the mixed density + Hartree path keeps V_ne in FP64): 21 fixed
+pairs within 2e-6, the accumulated rounding of 42 sweeps stored in FP32.  Inputs are printed before
+each GPU step.
 """
 import os
 import warnings

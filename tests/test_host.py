@@ -88,6 +88,28 @@ def test_save_load_pickle(tmp_path):
     assert h.origin["y"] == 2.0 and h.step == g.step
 
 
+class _Planted:
+    """Unpickles to a call of os.getcwd: the kind of global Grid.Load must refuse."""
+    def __reduce__(self):
+        import os
+        return (os.getcwd, ())
+
+
+def test_load_pickle_refuses_foreign_globals(tmp_path):
+    import pickle
+    from qmtools import Grid
+    path = str(tmp_path / "bad.pkl")
+    with open(path, "wb") as fh:
+        pickle.dump({"shape": (1, 8, 8, 8), "origin": None, "step": 0.1, "lattice": None,
+                     "qube": _Planted()}, fh)
+    with pytest.raises(pickle.UnpicklingError, match="refused pickle global"):
+        Grid.Load(None, path)
+    with open(path, "wb") as fh:
+        pickle.dump([1, 2, 3], fh)
+    with pytest.raises(ValueError, match="not a grid"):
+        Grid.Load(None, path)
+
+
 def test_field_xyz_is_x_fastest():
     from qmtools import Grid
     g = Grid((1, 8, 16, 24), 0.1)

@@ -422,3 +422,16 @@ def test_vne_loop_runs_one_pair_below_zero_cap():
     res, n, delta, conv = ov.vne_diffusion([1.0], [(0.5, 0.5, 0.5)], 6, 6, 6, 0.2, (0, 0, 0), tolerance=-1.0,
                                            maxiters=-0.5)
     assert n == 1 and not conv and res.any()
+
+
+@pytest.mark.parametrize("shape,sweeps,held", [((11, 9, 13), 0, True), ((11, 9, 13), 1, True),
+                                               ((11, 9, 13), 17, True), ((16, 8, 10), 40, False)])
+def test_jacobi_closed_form_equals_sweeps(shape, sweeps, held):
+    """The spectral form used for 2000-sweep comparisons equals the plain sweep loop."""
+    nx, ny, nz = shape
+    rng = numpy.random.default_rng(sweeps + 3)
+    rho = rng.standard_normal(nx * ny * nz)
+    B = numpy.where(oj.boundary_mask(nx, ny, nz), rng.standard_normal(nx * ny * nz), 0.0) if held else None
+    a = oj.jacobi(rho, nx, ny, nz, 0.3, sweeps, bvals=B)
+    b = oj.jacobi_closed_form(rho, nx, ny, nz, 0.3, sweeps, bvals=B)
+    assert numpy.abs(a - b).max() <= 1e-14 * max(numpy.abs(a).max(), 1.0)

@@ -77,6 +77,9 @@ int qm_ctx_create(int device, qm_ctx** out) {
   qm_ctx* c = new (std::nothrow) qm_ctx();
   if (!c) return QM_ENOMEM;
   c->device = device;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0) c->ncu = ncu;
+  (void)hipGetLastError();
   if (hipSetDevice(device) != hipSuccess ||
       hipHostMalloc((void**)&c->host_scalars, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -149,6 +152,8 @@ static void free_mol_device(qm_mol* m) {
   if (m->prims) (void)hipFree(m->prims);
   if (m->prims_l2e) (void)hipFree(m->prims_l2e);
   m->prims_l2e = nullptr;
+  if (m->kgroups) (void)hipFree(m->kgroups);
+  m->kgroups = nullptr;
   if (m->P64) (void)hipFree(m->P64);
   if (m->M64) (void)hipFree(m->M64);
   if (m->P16) (void)hipFree(m->P16);
@@ -217,6 +222,13 @@ int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr,
   std::vector<double4> at(natoms);
   for (int a = 0; a < natoms; ++a)
     at[a] = make_double4(xyz_bohr[3 * a], xyz_bohr[3 * a + 1], xyz_bohr[3 * a + 2], (double)Z[a]);
+  // where each group of 8 basis functions starts: (shell, m); groups past nbf hold (nshell, 0)
+  std::vector<int2> kg(m->nbf_pad / 8, make_int2(nshell, 0));
+  for (int s = 0; s < nshell; ++s)
+    for (int c = 0; c < 2 * m->h_shells[s].L + 1; ++c) {
+      const int f = m->h_shells[s].bf + c;
+      if (f % 8 == 0) kg[f / 8] = make_int2(s, c);
+    }
   std::vector<double2> pr(nprim);
   std::vector<float2> pr32(nprim);
   for (int p = 0; p < nprim; ++p) {
@@ -227,7 +239,8 @@ int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr,
   if (hipMalloc(&m->atoms, sizeof(double4) * natoms) != hipSuccess ||
       hipMalloc(&m->shells, sizeof(DevShell) * nshell) != hipSuccess ||
       hipMalloc(&m->prims, sizeof(double2) * nprim) != hipSuccess ||
-      hipMalloc(&m->prims_l2e, sizeof(float2) * nprim) != hipSuccess) {
+      hipMalloc(&m->prims_l2e, sizeof(float2) * nprim) != hipSuccess ||
+      hipMalloc(&m->kgroups, sizeof(int2) * kg.size()) != hipSuccess) {
     (void)hipGetLastError();
     free_mol_device(m);
     delete m;
@@ -236,7 +249,8 @@ int qm_mol_create(qm_ctx* ctx, int natoms, const int* Z, const double* xyz_bohr,
   if (hipMemcpy(m->atoms, at.data(), sizeof(double4) * natoms, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->shells, m->h_shells.data(), sizeof(DevShell) * nshell, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->prims, pr.data(), sizeof(double2) * nprim, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(m->prims_l2e, pr32.data(), sizeof(float2) * nprim, hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(m->prims_l2e, pr32.data(), sizeof(float2) * nprim, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->kgroups, kg.data(), sizeof(int2) * kg.size(), hipMemcpyHostToDevice) != hipSuccess) {
     hipError_t e = hipGetLastError();
     free_mol_device(m);
     delete m;

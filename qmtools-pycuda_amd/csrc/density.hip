@@ -46,6 +46,19 @@ constexpr double C2A = 1.0925484305920792;
 constexpr double C20 = 0.31539156525252005;
 constexpr double C22 = 0.5462742152960396;
 
+// Roundings fixed by explicit FMAs (no choice left to contraction), so that every kernel that
+// evaluates FP32 basis values (written to memory or evaluated inside the contraction) gets the
+// same bits: squared distance in FP64, 2z^2 - x^2 - y^2 as (z-x)(z+x) + (z-y)(z+y), radial sums.
+__device__ __forceinline__ double dist2(double ex, double ey, double ez) {
+  return __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
+}
+__device__ __forceinline__ float d0_factor(float x, float y, float z) {
+  return __builtin_fmaf(z - x, z + x, (z - y) * (z + y));
+}
+__device__ __forceinline__ double d0_factor(double x, double y, double z) {
+  return __builtin_fma(z - x, z + x, (z - y) * (z + y));
+}
+
 // Values of the 2L+1 functions of a shell, m order (0, +1, -1, +2, -2).
 template <typename R>
 __device__ __forceinline__ int shell_values(int L, R x, R y, R z, R radial, R* v) {
@@ -61,7 +74,7 @@ __device__ __forceinline__ int shell_values(int L, R x, R y, R z, R radial, R* v
   }
   // 2z^2 - x^2 - y^2 and x^2 - y^2 as products of sums and differences: no cancellation
   // where they pass through zero (in FP32 the plain forms lose up to ulp(r^2) there)
-  v[0] = (R)C20 * ((z - x) * (z + x) + (z - y) * (z + y)) * radial;
+  v[0] = (R)C20 * d0_factor(x, y, z) * radial;
   v[1] = (R)C2A * x * z * radial;
   v[2] = (R)C2A * z * y * radial;
   v[3] = (R)C22 * ((x - y) * (x + y)) * radial;
@@ -201,13 +214,13 @@ __device__ __forceinline__ void phi_f32_body(int blk, const MolView& m, const fl
       dx = (float)ex;
       dy = (float)ey;
       dz = (float)ez;
-      r2 = (float)(ex * ex + ey * ey + ez * ez);
+      r2 = (float)dist2(ex, ey, ez);
     }
     float radial = 0.f;
     if (r2 <= sh.rcut2) {
       for (int p = 0; p < sh.nprim; ++p) {
         const float2 pc = prim_l2e[sh.poff + p];
-        radial += pc.y * __builtin_amdgcn_exp2f(-pc.x * r2);
+        radial = __builtin_fmaf(pc.y, __builtin_amdgcn_exp2f(-pc.x * r2), radial);
       }
     }
     float* __restrict__ col = phi + (int64_t)sh.bf * ld + t;
@@ -248,6 +261,9 @@ template <typename T16>
 __device__ __forceinline__ void phi_split_body(int blk, const MolView& m, const float2* __restrict__ prim_l2e,
                                                const GeomView& g, int64_t v0, int nvox, int nsub, int npts, int ld,
                                                int nbf_pad, float S, T16* __restrict__ hi, T16* __restrict__ lo) {
+  // no contraction: hi is the 16-bit rounding of the FP32 value (a product folded into the
+  // conversion would round once from the exact product), identical in the fused kernel
+#pragma clang fp contract(off)
   __shared__ uint32_t stage[8][kPhiThreads];
   const int t = blk * kPhiThreads + threadIdx.x;
   if (t >= ld) return;
@@ -291,13 +307,13 @@ __device__ __forceinline__ void phi_split_body(int blk, const MolView& m, const 
       dx = (float)ex;
       dy = (float)ey;
       dz = (float)ez;
-      r2 = (float)(ex * ex + ey * ey + ez * ez);
+      r2 = (float)dist2(ex, ey, ez);
     }
     float radial = 0.f;
     if (r2 <= sh.rcut2) {
       for (int p = 0; p < sh.nprim; ++p) {
         const float2 pc = prim_l2e[sh.poff + p];
-        radial += pc.y * __builtin_amdgcn_exp2f(-pc.x * r2);
+        radial = __builtin_fmaf(pc.y, __builtin_amdgcn_exp2f(-pc.x * r2), radial);
       }
     }
     // S is a power of two: scaling the radial part scales every value exactly
@@ -1180,6 +1196,380 @@ __global__ __launch_bounds__(512) void k_contract_split_dma(const void* __restri
   tiled_dma_body<T16>(rt, cb, (const T16*)phi, ld, p_hi, p_lo, np, nbf, ncb, part, ld);
 }
 
+// ---------------------------------------------------------------------------
+// Fused split density (split variant 4): the basis values are
+// evaluated inside the contraction kernel, stage by stage, straight into LDS,
+// so they never travel through HBM (the two-kernel path writes and re-reads
+// 4 bytes per function and point).
+//
+// A workgroup of 8 waves owns 128 points and one column group of 320 columns
+// (20 tiles of 16), right-aligned on nbf rounded up to 32 like the DMA tiles
+// (only group 0 holds padding, as columns below 0).  K runs in stages of 32
+// over k < c0 + 320 (M = D + 2U is upper triangular).  The waves split by role:
+//
+//   * waves 4..7 (producers, one per SIMD) evaluate the basis values of the
+//     next stage: wave 4 + g takes the 8 functions of k group g for all 128
+//     points (two per lane), so the shells it visits are wave-uniform; a
+//     per-molecule table gives the shell and m of the group's first function.
+//     Values are scaled by S, split into 16-bit hi + lo with the roundings of
+//     phi_split_body, and stored as one 16-byte word per plane and point into
+//     the next slot of the basis-value ring.  The producers also bring the
+//     next stage of M (hi and lo, 32 k x 320 columns) into the M ring, loading
+//     it into registers before the evaluation and storing it after;
+//   * waves 0..3 (consumers, one per SIMD) run the MFMAs of the current
+//     stage: wave w owns the 5 tiles w, w + 4, .., w + 16 of the group (the
+//     column tiles are dealt round-robin so that the tiles that drop out of
+//     the triangle leave every wave evenly) and all 8 row tiles, 160 FP32
+//     accumulators.  X = Mh Phl + Ml Phh + Mh Phh per tile (M first: the tile
+//     holds X transposed), and the row-dot of a tile is summed in the stage
+//     that completes it, from the basis values in the ring (as in
+//     tiled_dma_body).
+//
+// One barrier per stage hands the two ring slots over; a producer and a
+// consumer share each SIMD, so the evaluation (VALU, exp2) of stage st + 1
+// runs beside the MFMAs of stage st.  With one producer wave per SIMD nothing
+// hides its load latency, so the molecule's tables (nuclei, shells, primitives,
+// function groups) are copied into LDS when the workgroup starts and every
+// table read during the stages is an LDS read.
+// LDS: 32 KB basis values + 80 KB M + 4 KB row-dot + up to 40 KB of tables.
+//
+// Measured on the 60-atom, 256^3 workload: 68 ms per grid against 27 ms for
+// basis values written to memory plus the DMA tiles (variant 3).  The single
+// producer wave per SIMD is the bound: its evaluation (LDS table reads in a
+// dependent chain per shell, FP64 offsets, per-slot selection of the harmonic)
+// takes about 3 us per stage, against well under 1 us of MFMAs; with the
+// evaluation switched off the kernel takes 24 ms.  Kept as a variant (no
+// basis-value buffer at all) and timed by the autotuner.
+constexpr int kFP = 128;          // points per workgroup
+constexpr int kFCT = 20;          // 16-column tiles per column group
+constexpr int kFC = 16 * kFCT;    // columns per column group
+constexpr int kFChunk = 1 << 24;  // points per launch (partial sums: 8 bytes x column groups each)
+constexpr int kFTab = 40960;      // bytes of LDS for the molecule's tables
+
+inline int fused_groups(int nbf) { return ((nbf + 31) / 32 * 32 + kFC - 1) / kFC; }
+
+// Byte offsets of the molecule tables in LDS: shells (int4 atom, L, nprim, first primitive),
+// nuclei (3 doubles), primitives (float2 alpha log2 e, coeff), function groups (int2), cutoffs.
+struct FusedTab {
+  int at, pr, kg, rc, bytes;
+};
+__host__ __device__ inline FusedTab fused_tab(int natoms, int nshell, int nprim, int nkg) {
+  FusedTab t;
+  t.at = 16 * nshell;
+  t.pr = t.at + 24 * natoms;
+  t.kg = t.pr + 8 * nprim;
+  t.rc = t.kg + 8 * nkg;
+  t.bytes = (t.rc + 4 * nshell + 15) / 16 * 16;
+  return t;
+}
+inline bool fused_fits(const qm_mol* m) {
+  return fused_tab(m->natoms, m->nshell, m->nprim, m->nbf_pad / 8).bytes <= kFTab;
+}
+
+// Solid-harmonic factor of function m of an L shell (m order 0, +1, -1, +2, -2): the
+// factor that shell_values multiplies by the radial part, with the same operations.
+__device__ __forceinline__ float harm_factor(int L, int m, float x, float y, float z) {
+  if (L == 0) return (float)C00;
+  if (L == 1) return m == 0 ? (float)C1 * z : (m == 1 ? (float)C1 * x : (float)C1 * y);
+  switch (m) {
+    case 0: return (float)C20 * d0_factor(x, y, z);
+    case 1: return (float)C2A * x * z;
+    case 2: return (float)C2A * z * y;
+    case 3: return (float)C22 * ((x - y) * (x + y));
+    default: return (float)C2A * x * y;
+  }
+}
+
+__device__ __forceinline__ int cdiv_floor0(int a, int b) {   // max(0, ceil(a / b)) for b > 0
+  return a <= 0 ? 0 : (a + b - 1) / b;
+}
+
+// Copies the molecule's tables into the workgroup's LDS (all 512 threads; caller synchronises).
+__device__ __forceinline__ void fused_load_tables(unsigned char* tab, const MolView& m,
+                                                  const float2* __restrict__ l2e, const int2* __restrict__ kgt,
+                                                  int nkg) {
+  const FusedTab o = fused_tab(m.natoms, m.nshell, m.nprim, nkg);
+  for (int i = threadIdx.x; i < m.nshell; i += blockDim.x) {
+    const DevShell sh = m.shells[i];
+    ((int4*)tab)[i] = make_int4(sh.atom, sh.L, sh.nprim, sh.poff);
+    ((float*)(tab + o.rc))[i] = sh.rcut2;
+  }
+  for (int i = threadIdx.x; i < m.natoms; i += blockDim.x) {
+    const double4 a = m.atoms[i];
+    double* d = (double*)(tab + o.at) + 3 * i;
+    d[0] = a.x;
+    d[1] = a.y;
+    d[2] = a.z;
+  }
+  for (int i = threadIdx.x; i < m.nprim; i += blockDim.x) ((float2*)(tab + o.pr))[i] = l2e[i];
+  for (int i = threadIdx.x; i < nkg; i += blockDim.x) ((int2*)(tab + o.kg))[i] = kgt[i];
+}
+
+template <typename T16>
+__device__ __forceinline__ void fused_tile(int rt, int cg, const unsigned char* tab, const MolView& m,
+                                           const GeomView& g, int64_t v0, int nvox, int nsub, int npts,
+                                           const T16* __restrict__ p_hi, const T16* __restrict__ p_lo, int np,
+                                           int nbf, int ncg, float S, double* __restrict__ part, int ldp) {
+  typedef typename Vec8<T16>::type V8;
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  __shared__ u32x4 phs[2][2][4][kFP];   // basis values: [slot][hi, lo][k group][point]
+  __shared__ u32x4 mrs[2][2][4][kFC];   // M: [slot][hi, lo][k group][column]
+  __shared__ double red[4][kFP];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int ne = (nbf + 31) / 32 * 32;
+  const int c0 = ne - (ncg - cg) * kFC;   // first column of the group (< 0 only in group 0)
+  const int nst = (c0 + kFC) / 32;
+  const int row0 = rt * kFP;
+  const int li = lane & 15, lk = lane >> 4;
+
+  if (wave >= 4) {
+    // ------------------------------------------------------------ producers
+    const FusedTab o = fused_tab(m.natoms, m.nshell, m.nprim, np / 8);
+    const int4* tsh = (const int4*)tab;
+    const double* tat = (const double*)(tab + o.at);
+    const float2* tpr = (const float2*)(tab + o.pr);
+    const int2* tkg = (const int2*)(tab + o.kg);
+    const float* trc = (const float*)(tab + o.rc);
+    const int pw = wave - 4;   // k group of every stage handled by this wave
+    double px[2], py[2], pz[2];
+    float sc[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int tt = row0 + lane + 64 * i;
+      px[i] = py[i] = pz[i] = 0.0;
+      if (tt < npts) chunk_point(g, v0, nvox, nsub, tt, px[i], py[i], pz[i]);
+      sc[i] = tt < npts ? S : 0.f;   // padding points store zeros
+    }
+    // M of a stage goes through registers, requested two stages ahead (one register set per stage
+    // in flight, set st & 1), so its load latency has the evaluation of two stages to complete
+    // (the stage loop below is unrolled by two so that the set index is a constant)
+    u32x4 mst[2][2][kFC / 64];
+    auto load_m = [&](int st, auto set) __attribute__((always_inline)) {
+      constexpr int R = decltype(set)::value;
+      if (st >= nst) return;
+      const int64_t krow = (int64_t)(4 * st + pw) * np;
+#pragma unroll
+      for (int c = 0; c < kFC / 64; ++c) {
+        const int col = max(c0 + 64 * c + lane, 0);   // columns below 0 belong to skipped tiles
+        mst[R][0][c] = *(const u32x4*)(p_hi + (krow + col) * 8);
+        mst[R][1][c] = *(const u32x4*)(p_lo + (krow + col) * 8);
+      }
+    };
+    auto store_m = [&](int st, auto set) __attribute__((always_inline)) {
+      constexpr int R = decltype(set)::value;
+      const int slot = st & 1;
+#pragma unroll
+      for (int c = 0; c < kFC / 64; ++c) {
+        mrs[slot][0][pw][64 * c + lane] = mst[R][0][c];
+        mrs[slot][1][pw][64 * c + lane] = mst[R][1][c];
+      }
+    };
+    const std::integral_constant<int, 0> set0;
+    const std::integral_constant<int, 1> set1;
+    auto make_phi = [&](int st) __attribute__((always_inline)) {
+#pragma clang fp contract(off)   // the roundings of phi_split_body
+      const int slot = st & 1;
+      const int2 e = tkg[4 * st + pw];
+      int s = __builtin_amdgcn_readfirstlane(e.x), mm = __builtin_amdgcn_readfirstlane(e.y);
+      int L = 0, atom = -1;
+      float dx[2], dy[2], dz[2], r2[2], rad[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) dx[i] = dy[i] = dz[i] = r2[i] = rad[i] = 0.f;
+      auto open_shell = [&]() __attribute__((always_inline)) {
+        const int4 sh = tsh[s];
+        const int sa = __builtin_amdgcn_readfirstlane(sh.x);
+        const int np_ = __builtin_amdgcn_readfirstlane(sh.z), po = __builtin_amdgcn_readfirstlane(sh.w);
+        const float rc = trc[s];
+        L = __builtin_amdgcn_readfirstlane(sh.y);
+        if (sa != atom) {   // r^2 in FP64, rounded once (as phi_split_body)
+          atom = sa;
+          const double ax = tat[3 * atom], ay = tat[3 * atom + 1], az = tat[3 * atom + 2];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            const double ex = px[i] - ax, ey = py[i] - ay, ez = pz[i] - az;
+            dx[i] = (float)ex;
+            dy[i] = (float)ey;
+            dz[i] = (float)ez;
+            r2[i] = (float)dist2(ex, ey, ez);
+          }
+        }
+        // primitives outer, the two points inner; the next primitive is read while this one is used
+        float radial[2] = {0.f, 0.f};
+        if (r2[0] <= rc || r2[1] <= rc) {
+          float2 pc = tpr[po];
+          for (int p = 0; p < np_; ++p) {
+            const float2 nx = tpr[po + min(p + 1, np_ - 1)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+              radial[i] = __builtin_fmaf(pc.y, __builtin_amdgcn_exp2f(-pc.x * r2[i]), radial[i]);
+            pc = nx;
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) rad[i] = (r2[i] <= rc ? radial[i] : 0.f) * sc[i];   // S: a power of two
+      };
+      if (s < m.nshell) open_shell();
+      uint32_t hw[2][4], lw[2][4];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (s < m.nshell && mm == 2 * L + 1) {
+          ++s;
+          mm = 0;
+          if (s < m.nshell) open_shell();
+        }
+        const bool live = s < m.nshell;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const float xs = live ? harm_factor(L, mm, dx[i], dy[i], dz[i]) * rad[i] : 0.f;
+          const T16 hq = (T16)xs;
+          const T16 lq = (T16)(xs - (float)hq);
+          if (j & 1) {
+            hw[i][j >> 1] |= bits16(hq) << 16;
+            lw[i][j >> 1] |= bits16(lq) << 16;
+          } else {
+            hw[i][j >> 1] = bits16(hq);
+            lw[i][j >> 1] = bits16(lq);
+          }
+        }
+        ++mm;
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        phs[slot][0][pw][lane + 64 * i] = u32x4{hw[i][0], hw[i][1], hw[i][2], hw[i][3]};
+        phs[slot][1][pw][lane + 64 * i] = u32x4{lw[i][0], lw[i][1], lw[i][2], lw[i][3]};
+      }
+    };
+    load_m(0, set0);
+    load_m(1, set1);
+    make_phi(0);
+    store_m(0, set0);
+    load_m(2, set0);   // into the register set stage 0 has released
+    __syncthreads();
+    for (int st = 0; st < nst; st += 2) {   // stage st + 1 uses set 1, stage st + 2 set 0
+      if (st + 1 < nst) {
+        make_phi(st + 1);
+        store_m(st + 1, set1);
+        load_m(st + 3, set1);
+      }
+      __syncthreads();
+      if (st + 1 < nst) {
+        if (st + 2 < nst) {
+          make_phi(st + 2);
+          store_m(st + 2, set0);
+          load_m(st + 4, set0);
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();   // the consumers' row-dot partials are in red
+  } else {
+    // ------------------------------------------------------------ consumers
+    const int w = wave;
+    f4 acc[8][5];
+    double pr[8];   // row-dot partial sums of this lane: point 16 r + li, summed over its columns
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      pr[r] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) acc[r][c] = f4{0, 0, 0, 0};
+    }
+    // tile c of this wave: columns n0(c) = c0 + 16 (w + 4 c) .. + 15; it takes stage st while
+    // 32 st <= n0 + 15, and never if n0 < 0 (padding)
+    const int cpad = min(cdiv_floor0(-c0 - 16 * w, 64), 5);
+    // first live tile in stage st; non-decreasing in st, so the stages run in 5 phases of fixed
+    // first tile (a loop per phase keeps the accumulators in place across stages)
+    auto first_live = [&](int st) { return min(max(cdiv_floor0(32 * st - c0 - 15 - 16 * w, 64), cpad), 5); };
+    auto body = [&](int st, auto first) __attribute__((always_inline)) {
+      constexpr int CS = decltype(first)::value;
+      const int slot = st & 1;
+      // the first live tile completes in this stage when its columns lie in 32 st .. 32 st + 31
+      const int nfirst = c0 + 16 * (w + 4 * CS);
+      const bool done = nfirst < 32 * st + 32;
+      const int dc = (nfirst - 32 * st) >> 4;   // which half of the stage's k range holds its columns
+      V8 mh[5], ml[5];
+#pragma unroll
+      for (int c = CS; c < 5; ++c) {
+        mh[c] = __builtin_bit_cast(V8, mrs[slot][0][lk][16 * (w + 4 * c) + li]);
+        ml[c] = __builtin_bit_cast(V8, mrs[slot][1][lk][16 * (w + 4 * c) + li]);
+      }
+      // basis-value fragments one row tile ahead; the scheduling barrier after each row tile keeps
+      // the compiler from hoisting further loads (registers are taken by the 160 accumulators)
+      V8 ah = __builtin_bit_cast(V8, phs[slot][0][lk][li]);
+      V8 al = __builtin_bit_cast(V8, phs[slot][1][lk][li]);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        V8 ahn = ah, aln = al;
+        if (r + 1 < 8) {
+          ahn = __builtin_bit_cast(V8, phs[slot][0][lk][16 * (r + 1) + li]);
+          aln = __builtin_bit_cast(V8, phs[slot][1][lk][16 * (r + 1) + li]);
+        }
+#pragma unroll
+        for (int c = CS; c < 5; ++c) {   // transposed tiles: M first
+          acc[r][c] = Vec8<T16>::mma(mh[c], al, acc[r][c]);
+          acc[r][c] = Vec8<T16>::mma(ml[c], ah, acc[r][c]);
+          acc[r][c] = Vec8<T16>::mma(mh[c], ah, acc[r][c]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        ah = ahn;
+        al = aln;
+      }
+      if (done) {
+        // lane (li, lk), register q of the tile is X[t = 16 r + li][n = nfirst + 4 lk + q]; the 4 basis
+        // values of that point sit in one 8-byte word of each plane of k group 2 dc + lk / 2
+        typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+        typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+        typedef typename std::conditional<std::is_same<T16, _Float16>::value, h4, b4>::type V4;
+        const int kg = 2 * dc + (lk >> 1);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const V4 h = __builtin_bit_cast(V4, ((const uint2*)&phs[slot][0][kg][16 * r + li])[lk & 1]);
+          const V4 l = __builtin_bit_cast(V4, ((const uint2*)&phs[slot][1][kg][16 * r + li])[lk & 1]);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) pr[r] += (double)(acc[r][CS][q] * ((float)h[q] + (float)l[q]));
+        }
+      }
+    };
+    __syncthreads();
+    int st = 0;
+    auto phase = [&](auto first) __attribute__((always_inline)) {
+      for (; st < nst && first_live(st) == decltype(first)::value; ++st) {
+        body(st, first);
+        __syncthreads();
+      }
+    };
+    phase(std::integral_constant<int, 0>());
+    phase(std::integral_constant<int, 1>());
+    phase(std::integral_constant<int, 2>());
+    phase(std::integral_constant<int, 3>());
+    phase(std::integral_constant<int, 4>());
+    for (; st < nst; ++st) __syncthreads();   // no live tile left in this wave
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      double p = pr[r];
+      p += __shfl_xor(p, 16);
+      p += __shfl_xor(p, 32);
+      if (lk == 0) red[w][16 * r + li] = p;
+    }
+    __syncthreads();
+  }
+  if (t < kFP) part[(int64_t)cg * ldp + row0 + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+}
+
+template <typename T16>
+__global__ __launch_bounds__(512) void k_density_fused(MolView m, const float2* __restrict__ l2e,
+                                                       const int2* __restrict__ kgt, GeomView g, int64_t v0, int nvox,
+                                                       int nsub, int npts, const T16* __restrict__ p_hi,
+                                                       const T16* __restrict__ p_lo, int np, int nbf, int nrt, int ncg,
+                                                       float S, double* __restrict__ part, int ldp) {
+  __shared__ __attribute__((aligned(16))) unsigned char tab[kFTab];
+  int rt, cg;
+  if (!tile_coords(blockIdx.x, nrt, ncg, rt, cg)) return;
+  fused_load_tables(tab, m, l2e, kgt, np / 8);
+  __syncthreads();
+  fused_tile<T16>(rt, cg, tab, m, g, v0, nvox, nsub, npts, p_hi, p_lo, np, nbf, ncg, S, part, ldp);
+}
+
 // Variant 3: tiles of 128 points x 64 columns with waves stacked 8 x 1 (wave w
 // owns rows 16w .. 16w+15 and all 64 columns), so every wave has the same
 // columns and, in the stages that cross the block's diagonal, all waves skip
@@ -1693,8 +2083,61 @@ void launch_split_tiles(hipStream_t s, int v, int blocks, const void* phi, int l
   }
 }
 
+// Split variant 4: basis values evaluated inside the contraction (k_density_fused), no basis-value
+// workspace; launches of up to kFChunk points, partial sums per column group.
+template <typename T16, typename TO>
+int density_fused(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int nsub, TO* out) {
+  const int np = mol->nbf_pad;
+  const int nsub3 = nsub * nsub * nsub;
+  const int64_t nvox_tot = npoints(g);
+  int64_t nvox = std::max<int64_t>(kFChunk / nsub3, 1);
+  nvox = std::min<int64_t>(nvox, nvox_tot);
+  const int64_t ldmax = round_up(nvox * nsub3, kFP);
+  const int ncg = fused_groups(mol->nbf);
+  int rc = QM_OK;
+  double* rho = nullptr;
+  if (nsub3 > 1 && (rc = qm_ws(ctx, s, WS_RHO, sizeof(double) * ldmax, (void**)&rho))) return rc;
+  double* part = nullptr;
+  if ((rc = qm_ws(ctx, s, WS_PART, sizeof(double) * (size_t)ncg * ldmax, (void**)&part))) return rc;
+  const T16* Phi = (const T16*)(std::is_same<T16, _Float16>::value ? mol->P16 : mol->Pb16);
+  const T16* Plo = Phi + (int64_t)np * np;
+  const float S = split_phi_scale(mol);
+  const GeomView gv = geom_view(g);
+  const MolView mv = mol->view();
+  // both factors of the row-dot carry S; divided out with the volume element
+  const double sc = g->step * g->step * g->step / mol->pfac / ((double)S * S);
+  for (int64_t v0 = 0; v0 < nvox_tot; v0 += nvox) {
+    const int nv = (int)std::min<int64_t>(nvox, nvox_tot - v0);
+    const int npts = nv * nsub3;
+    const int ld = (int)round_up(npts, kFP);
+    const int nrt = ld / kFP;
+    {
+      ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
+      hipLaunchKernelGGL(k_density_fused<T16>, dim3(tile_grid(nrt, ncg)), dim3(512), 0, s, mv,
+                         (const float2*)mol->prims_l2e, (const int2*)mol->kgroups, gv, v0, nv, nsub, npts, Phi, Plo,
+                         np, mol->nbf, nrt, ncg, S, part, ld);
+    }
+    QM_CHECK_LAUNCH(ctx);
+    if (nsub3 == 1) {
+      hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncg,
+                         npts, sc, (const float*)nullptr, out, v0);
+    } else {
+      hipLaunchKernelGGL(k_tiles_finish<double>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld,
+                         ncg, npts, sc / nsub3, (const float*)nullptr, rho, (int64_t)0);
+      QM_CHECK_LAUNCH(ctx);
+      hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv, nsub3,
+                         out, v0);
+    }
+    QM_CHECK_LAUNCH(ctx);
+  }
+  return QM_OK;
+}
+
 template <typename T16, typename TO>
 int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int nsub, TO* out) {
+  if (ctx->split_tm == 4 && fused_fits(mol)) return density_fused<T16, TO>(ctx, mol, s, g, nsub, out);
+  // tables too large for the fused kernel's LDS: basis values written pre-split, LDS DMA tiles
+  const int variant = ctx->split_tm == 4 ? 3 : ctx->split_tm;
   const int np = mol->nbf_pad;
   const int nsub3 = nsub * nsub * nsub;
   const int64_t nvox_tot = npoints(g);
@@ -1717,8 +2160,8 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
   }
   const T16* Phi = (const T16*)(std::is_same<T16, _Float16>::value ? mol->P16 : mol->Pb16);
   const T16* Plo = Phi + (int64_t)np * np;
-  const bool pre = split_pre(ctx->split_tm);
-  const bool tiled = split_tiled(ctx->split_tm);
+  const bool pre = split_pre(variant);
+  const bool tiled = split_tiled(variant);
   const float S = split_phi_scale(mol);
   const int ncb = (np + kTT - 1) / kTT;
   double* part = nullptr;
@@ -1745,7 +2188,7 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
     ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
     if (tiled) {
       const int nrt = ld / kTT;
-      launch_split_tiles<T16>(s, ctx->split_tm, tile_grid(nrt, ncb), phi, ld, pscale, Phi, Plo, np, mol->nbf, nrt, ncb,
+      launch_split_tiles<T16>(s, variant, tile_grid(nrt, ncb), phi, ld, pscale, Phi, Plo, np, mol->nbf, nrt, ncb,
                               part);
       QM_CHECK_LAUNCH(ctx);
       // pre-split: both factors of the row-dot carry S, divided out here with the volume element
@@ -1763,10 +2206,10 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
       }
     } else if (nsub3 == 1) {
       launch_contract_split<T16, TO>(s, ld, (const float*)phi, pscale, Phi, Plo, np, npts, h3, out, v0,
-                                     ctx->split_tm);
+                                     variant);
     } else {
       launch_contract_split<T16, double>(s, ld, (const float*)phi, pscale, Phi, Plo, np, npts, h3 / nsub3, rho,
-                                         (int64_t)0, ctx->split_tm);
+                                         (int64_t)0, variant);
       QM_CHECK_LAUNCH(ctx);
       hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv,
                          nsub3, out, v0);
@@ -1850,8 +2293,8 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
   int rc = qm_enter(ctx);
   if (rc) return rc;
   if (knob == QM_TUNE_SPLIT_VARIANT) {
-    QM_REQUIRE(ctx, split_tiled(value) || value == 64 || value == 128, QM_EINVAL,
-               "tune: split variant must be 0..3 (tiles) or 64 / 128 (rows per workgroup)");
+    QM_REQUIRE(ctx, split_tiled(value) || value == 4 || value == 64 || value == 128, QM_EINVAL,
+               "tune: split variant must be 0..3 (tiles), 4 (fused) or 64 / 128 (rows per workgroup)");
     ctx->split_tm = value;
     return QM_OK;
   }
@@ -1900,7 +2343,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   void* out = nullptr;
   if ((rc = qm_ws(ctx, s, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
-  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 64, 128} : std::vector<int>{0, 1, 2, 3, 4};
+  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 4, 64, 128} : std::vector<int>{0, 1, 2, 3, 4};
   int& knob = split ? ctx->split_tm : ctx->f64_variant;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));
@@ -1971,6 +2414,7 @@ struct BChunk {
   int ncb;
   int blk0_phi;         // first block in the round's basis-value (and tiled finish) launch
   int blk0;             // first block in its contraction launch
+  const int2* kg;       // fused kernel: the molecule's function-group table
 };
 
 __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ c, int n, int b, bool phi_launch) {
@@ -2115,6 +2559,18 @@ void launch_batch_split_tiles(hipStream_t s, int v, int blocks, const BChunk* c,
   }
 }
 
+template <typename T16>
+__global__ __launch_bounds__(512) void kb_density_fused(const BChunk* __restrict__ c, int n) {
+  __shared__ __attribute__((aligned(16))) unsigned char tab[kFTab];
+  const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
+  int rt, cg;
+  if (!tile_coords(blockIdx.x - q.blk0, q.ld / kFP, q.ncb, rt, cg)) return;
+  fused_load_tables(tab, q.mol, q.l2e, q.kg, q.nbf_pad / 8);
+  __syncthreads();
+  fused_tile<T16>(rt, cg, tab, q.mol, q.g, q.v0, q.nvox, 1, q.nvox, (const T16*)q.ph, (const T16*)q.pl, q.nbf_pad,
+                  q.mol.nbf, q.ncb, q.phis, q.part, q.ld);
+}
+
 // same sums as k_tiles_finish; one thread per point, blocks numbered like the basis-value launch
 template <typename TO>
 __global__ __launch_bounds__(kPhiThreads) void kb_tiles_finish(const BChunk* __restrict__ c, int n, int split) {
@@ -2134,6 +2590,105 @@ struct Round {
   int phi_blocks;
 };
 
+// Charge normalisation of every molecule of a batch: Q <- Q sum Z / sum Q (device-side factors).
+int batch_normalize(qm_ctx* ctx, hipStream_t s, int nmol, qm_mol* const* mols, const qm_geom* geoms,
+                    void* const* outs, int out_dtype, int normalize) {
+  if (!normalize) return QM_OK;
+  double* tot = nullptr;
+  int rc = QM_OK;
+  if ((rc = qm_ws(ctx, s, WS_BATCH2, sizeof(double) * (size_t)nmol, (void**)&tot))) return rc;
+  for (int i = 0; i < nmol; ++i) {
+    if ((rc = qm_reduce_dev(ctx, s, QM_RED_SUM, out_dtype, outs[i], nullptr, npoints(&geoms[i]), tot + i))) return rc;
+    if ((rc = qm_scale_dev(ctx, s, out_dtype, outs[i], npoints(&geoms[i]), nullptr, mols[i]->qtot, tot + i)))
+      return rc;
+  }
+  return QM_OK;
+}
+
+// Batched split variant 4: chunks of every molecule in rounds whose partial sums fit kFPartBudget;
+// one fused launch and one finish launch per round.
+constexpr size_t kFPartBudget = (size_t)1 << 30;
+
+template <typename T16>
+int density_batch_fused(qm_ctx* ctx, hipStream_t s, int nmol, qm_mol* const* mols, const qm_geom* geoms,
+                        void* const* outs, int out_dtype) {
+  const bool half = std::is_same<T16, _Float16>::value;
+  std::vector<BChunk> table;
+  std::vector<std::pair<size_t, size_t>> rounds;   // [first, count) of each round
+  std::vector<std::pair<int, int>> round_blocks;   // (fused blocks, finish blocks)
+  size_t part = 0, first = 0, max_part = 0;
+  int blocks = 0, fblocks = 0;
+  auto flush = [&]() {
+    if (table.size() == first) return;
+    rounds.push_back({first, table.size() - first});
+    round_blocks.push_back({blocks, fblocks});
+    max_part = std::max(max_part, part);
+    first = table.size();
+    part = 0;
+    blocks = 0;
+    fblocks = 0;
+  };
+  for (int i = 0; i < nmol; ++i) {
+    qm_mol* m = mols[i];
+    const int np = m->nbf_pad;
+    const int64_t N = npoints(&geoms[i]);
+    const float S = split_phi_scale(m);
+    const int ncg = fused_groups(m->nbf);
+    const uint16_t* base = (const uint16_t*)(half ? m->P16 : m->Pb16);
+    const double h3 = geoms[i].step * geoms[i].step * geoms[i].step;
+    for (int64_t v0 = 0; v0 < N; v0 += kBatchChunk) {
+      BChunk c{};
+      c.mol = m->view();
+      c.l2e = m->prims_l2e;
+      c.kg = m->kgroups;
+      c.g = geom_view(&geoms[i]);
+      c.v0 = v0;
+      c.nvox = (int)std::min<int64_t>(kBatchChunk, N - v0);
+      c.ld = (int)round_up(c.nvox, kFP);
+      c.nbf_pad = np;
+      c.ncb = ncg;
+      c.ph = base;
+      c.pl = base + (size_t)np * np;
+      c.phis = S;
+      c.scale = h3 / m->pfac / ((double)S * S);
+      c.out = outs[i];
+      const size_t bytes = sizeof(double) * (size_t)ncg * c.ld;
+      if (part + bytes > kFPartBudget) flush();
+      c.part = (double*)(uintptr_t)part;   // offset for now, rebased below
+      part += bytes;
+      c.blk0 = blocks;
+      blocks += tile_grid(c.ld / kFP, ncg);
+      c.blk0_phi = fblocks;
+      fblocks += (c.ld + kPhiThreads - 1) / kPhiThreads;
+      table.push_back(c);
+    }
+  }
+  flush();
+  void *pt = nullptr, *dtab = nullptr;
+  int rc = QM_OK;
+  if ((rc = qm_ws(ctx, s, WS_PART, std::max<size_t>(max_part, 16), &pt))) return rc;
+  if ((rc = qm_ws(ctx, s, WS_BATCH, sizeof(BChunk) * table.size(), &dtab))) return rc;
+  for (BChunk& c : table) c.part = (double*)((char*)pt + (size_t)(uintptr_t)c.part);
+  QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, table.data(), sizeof(BChunk) * table.size(), hipMemcpyHostToDevice, s));
+  QM_CHECK_HIP(ctx, hipStreamSynchronize(s));    // the host table is pageable and goes out of scope
+  const BChunk* dt = (const BChunk*)dtab;
+  for (size_t r = 0; r < rounds.size(); ++r) {
+    const BChunk* rc0 = dt + rounds[r].first;
+    const int rn = (int)rounds[r].second;
+    {
+      ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
+      hipLaunchKernelGGL(kb_density_fused<T16>, dim3(round_blocks[r].first), dim3(512), 0, s, rc0, rn);
+    }
+    QM_CHECK_LAUNCH(ctx);
+    if (out_dtype == QM_F64)
+      hipLaunchKernelGGL(kb_tiles_finish<double>, dim3(round_blocks[r].second), dim3(kPhiThreads), 0, s, rc0, rn, 0);
+    else
+      hipLaunchKernelGGL(kb_tiles_finish<float>, dim3(round_blocks[r].second), dim3(kPhiThreads), 0, s, rc0, rn, 0);
+    QM_CHECK_LAUNCH(ctx);
+  }
+  return QM_OK;
+}
+
 }  // namespace
 
 extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* const* mols, const qm_geom* geoms,
@@ -2151,11 +2706,20 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   }
   hipStream_t s = (hipStream_t)stream;
   const bool f64 = precision == QM_PREC_FP64;
+  bool all_fit = true;
+  for (int i = 0; i < nmol; ++i) all_fit = all_fit && fused_fits(mols[i]);
+  if (!f64 && ctx->split_tm == 4 && all_fit) {
+    rc = precision == QM_PREC_F16X3 ? density_batch_fused<_Float16>(ctx, s, nmol, mols, geoms, outs, out_dtype)
+                                    : density_batch_fused<__bf16>(ctx, s, nmol, mols, geoms, outs, out_dtype);
+    if (rc) return rc;
+    return batch_normalize(ctx, s, nmol, mols, geoms, outs, out_dtype, normalize);
+  }
   const size_t elt = f64 ? sizeof(double) : sizeof(float);
-  const int tm = f64 ? 64 : ctx->split_tm;
-  const bool tiled = f64 ? ctx->f64_variant >= 2 : split_tiled(ctx->split_tm);
+  const int sv = ctx->split_tm == 4 ? 3 : ctx->split_tm;   // fused variant without room for the tables
+  const int tm = f64 ? 64 : sv;
+  const bool tiled = f64 ? ctx->f64_variant >= 2 : split_tiled(sv);
   // basis values written scaled and split (same bytes as FP32)
-  const bool pre = !f64 && split_pre(ctx->split_tm);
+  const bool pre = !f64 && split_pre(sv);
   // chunks per molecule, packed into rounds of at most kPhiBudget bytes of basis values
   std::vector<BChunk> table;
   std::vector<Round> rounds;
@@ -2292,9 +2856,9 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
         else if (f64)
           hipLaunchKernelGGL(kb_contract_f64_tiles<false>, dim3(blocks), dim3(kTWG), 0, s, gc, gn);
         else if (precision == QM_PREC_F16X3)
-          launch_batch_split_tiles<_Float16>(s, ctx->split_tm, blocks, gc, gn);
+          launch_batch_split_tiles<_Float16>(s, sv, blocks, gc, gn);
         else
-          launch_batch_split_tiles<__bf16>(s, ctx->split_tm, blocks, gc, gn);
+          launch_batch_split_tiles<__bf16>(s, sv, blocks, gc, gn);
         QM_CHECK_LAUNCH(ctx);
         // finish over the whole round: chunk blocks numbered as in the basis-value launch
         const int per_point = !f64 && !pre;   // per-point scales to divide out
@@ -2317,14 +2881,5 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       QM_CHECK_LAUNCH(ctx);
     }
   }
-  if (normalize) {
-    double* tot = nullptr;
-    if ((rc = qm_ws(ctx, s, WS_BATCH2, sizeof(double) * (size_t)nmol, (void**)&tot))) return rc;
-    for (int i = 0; i < nmol; ++i) {
-      if ((rc = qm_reduce_dev(ctx, s, QM_RED_SUM, out_dtype, outs[i], nullptr, npoints(&geoms[i]), tot + i))) return rc;
-      if ((rc = qm_scale_dev(ctx, s, out_dtype, outs[i], npoints(&geoms[i]), nullptr, mols[i]->qtot, tot + i)))
-        return rc;
-    }
-  }
-  return QM_OK;
+  return batch_normalize(ctx, s, nmol, mols, geoms, outs, out_dtype, normalize);
 }

@@ -32,7 +32,7 @@ struct MolView {
   const double4* atoms;   // x, y, z (bohr), Z
   const DevShell* shells;
   const double2* prims;   // alpha, coeff
-  int natoms, nshell, nbf;
+  int natoms, nshell, nbf, nprim;
 };
 
 struct GeomView {
@@ -100,9 +100,10 @@ struct qm_ctx {
   bool prof = false;
   hipStream_t graph_stream = nullptr;   // non-blocking stream for graph capture (default priority)
   std::vector<LiveGraph> graphs;        // graphs whose replays may still run
-  int split_tm = 3;                     // split contraction variant (0..3 tiled, 64 / 128 all-columns)
+  int split_tm = 3;                     // split contraction variant (0..3 tiled, 4 fused, 64 / 128 all-columns)
   int f64_variant = 4;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled, 3 rows, 4 tiled + DMA
   int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)
+  int ncu = 256;                        // compute units of the device
   std::vector<ProfRec> recs;
 };
 
@@ -112,6 +113,7 @@ struct qm_mol {
   DevShell* shells = nullptr;
   double2* prims = nullptr;
   float2* prims_l2e = nullptr;      // (alpha log2 e, coeff) in FP32 for the FP32 basis kernel
+  int2* kgroups = nullptr;          // per 8 functions (nbf_pad / 8): shell of the first function, its m index
   // host copies (used to build per-request tables)
   std::vector<double> xyz, Z;
   std::vector<DevShell> h_shells;
@@ -133,7 +135,7 @@ struct qm_mol {
   MolView view() const {
     MolView v;
     v.atoms = atoms; v.shells = shells; v.prims = prims;
-    v.natoms = natoms; v.nshell = nshell; v.nbf = nbf;
+    v.natoms = natoms; v.nshell = nshell; v.nbf = nbf; v.nprim = nprim;
     return v;
   }
 };

@@ -27,7 +27,7 @@ def items():
                                                     ("fp64", "f64_variant", 3), ("fp64", "f64_variant", 4),
                                                     ("mixed", "split_variant", 0), ("mixed", "split_variant", 1),
                                                     ("mixed", "split_variant", 2), ("mixed", "split_variant", 3),
-                                                    ("mixed", "split_variant", 128)])
+                                                    ("mixed", "split_variant", 4), ("mixed", "split_variant", 128)])
 def test_density_batch_equals_single(items, precision, knob, variant):
     import qmtools
     from qmtools import autotune, batch

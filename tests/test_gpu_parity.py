@@ -122,9 +122,10 @@ def test_density_block_alignment(qm, obasis, natoms, seed):
 
 @pytest.mark.parametrize("prec,tol", [("f16", 1e-6), ("bf16", 2e-5)])
 @pytest.mark.parametrize("subgrid", [1, 2])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_density_mixed_tiled(qm, small, prec, tol, subgrid, variant):
-    """The 128 x 128 tiled split contraction (partial sums per column block) against the oracle."""
+    """The tiled split contractions (partial sums per column block; 4 evaluates the basis values inside
+    the contraction kernel) against the oracle."""
     from oracle import density as od
     from qmtools import autotune
     mol, grid, sysm = small
@@ -431,7 +432,7 @@ def test_density_fp64_large_basis(qm, large, variant):
     assert q.raw_total == pytest.approx(raw, rel=1e-10)
 
 
-@pytest.mark.parametrize("rows", [0, 1, 2, 3, 64, 128])
+@pytest.mark.parametrize("rows", [0, 1, 2, 3, 4, 64, 128])
 def test_density_f16_large_basis_tiles(qm, large, rows):
     from oracle import density as od
     from qmtools import autotune
@@ -459,7 +460,7 @@ def test_autotune_cache(qm, small, tmp_path):
     _say("autotune", grid.shape)
     old = autotune.get("split_variant")
     e = autotune.autotune_density(mol, grid, "mixed", path=path)
-    assert e["split_variant"] in (0, 1, 2, 3, 64, 128) and e["ms"] > 0
+    assert e["split_variant"] in (0, 1, 2, 3, 4, 64, 128) and e["ms"] > 0
     table = json.load(open(path))
     assert autotune.key_for(mol, "mixed") in table
     autotune.set("split_variant", 64)
@@ -530,3 +531,37 @@ def test_translate_reuses_resident_tables(qm, obasis):
     # a second GTO call reuses the rebuilt table
     qm.QMTools.Compute_hartree_points(mol, pts)
     assert dev.stats()["gto_builds"] == 2
+
+
+@pytest.mark.parametrize("natoms,seed,n", [(3, 7, 24), (12, 4, 16), (56, 96, 16), (120, 5, 8)])
+@pytest.mark.parametrize("prec", ["f16", "bf16"])
+def test_density_fused_equals_two_kernel_path(qm, natoms, seed, n, prec):
+    """The fused kernel (basis values evaluated into LDS) evaluates the basis values with the roundings
+    of the basis-value kernel and forms the same MFMA products in the same k order as the two-kernel
+    path (variant 3); the row-dot terms are grouped and rounded differently (measured differences
+    4e-10 and below, against 1e-7 for the split itself), so the gate is 1e-9: a wrong or missing
+    basis value or product would exceed it.  nbf 33 .. 1200: one to four column groups, padding columns
+    in group 0, and more than 1024 basis functions."""
+    import ctypes
+    import torch
+    from qmtools import _native as nat, autotune, synth
+    from qmtools.qmtools import device_molecule
+    mol = synth.synthetic_molecule(natoms, seed, qm.BasisSet(BASIS))
+    grid = synth.centred_grid(mol, n, step_bohr=0.6, pad_ang=2.0)
+    _say("fused vs two-kernel", prec, natoms, "nbf", mol.nbf, grid.shape)
+    g = nat.geom(grid)
+    code = nat.QM_PREC_F16X3 if prec == "f16" else nat.QM_PREC_BF16X3
+    outs = []
+    old = autotune.get("split_variant")
+    try:
+        for v in (3, 4):   # FP64 output: the FP32 rounding of the drop-in grid would hide the comparison
+            autotune.set("split_variant", v)
+            o = torch.empty(int(grid.npts), dtype=torch.float64, device="cuda")
+            nat.call("qm_density", device_molecule(mol).handle, nat.stream(), ctypes.byref(g), 1, code, nat.QM_F64,
+                     nat.dptr(o), 0, None)
+            outs.append(o.cpu().numpy())
+    finally:
+        autotune.set("split_variant", old)
+    err = _rel(outs[1], outs[0])
+    print(f"fused vs two-kernel {prec} nbf {mol.nbf}: {err:.2e}")
+    assert err <= 1e-9

@@ -140,6 +140,17 @@ int qm_basis_points(qm_ctx* ctx, qm_mol* mol, void* stream, const double* pts, i
 int qm_reduce(qm_ctx* ctx, void* stream, int op, int dtype, const void* a, const void* b,
               int64_t n, double* result);
 
+/* qm_reduce that allocates nothing: same operations, same kernels, the same bits in *result.
+ * No counterpart in the reference; for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid allocation).  The block partials and the device
+ * result live in a fixed file-scope device array of 16.6 KB per GPU (2048 partials + result),
+ * which holds what any n needs; n at or beyond 2^34 (the library's grid limit) is refused with
+ * QM_ELIMIT.  Every call on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * (other streams, other host threads) race; nothing in it needs to start at zero.  Synchronises
+ * `stream` like qm_reduce. */
+int qm_reduce_noalloc(qm_ctx* ctx, void* stream, int op, int dtype, const void* a, const void* b,
+                      int64_t n, double* result);
+
 /* a *= factor.  Replaces gpu_rescale (kernel_utils.cu:54-65), Compute_qscale (qmtools.py:2807-2812). */
 int qm_scale(qm_ctx* ctx, void* stream, int dtype, void* a, int64_t n, double factor);
 

@@ -86,6 +86,12 @@ int qm_ctx_create(int device, qm_ctx** out) {
     delete c;
     return QM_EHIP;
   }
+  // with the device current: where this GPU holds the allocation-free reduction's scratch
+  if (qm_noalloc_setup(device) != QM_OK) {
+    (void)hipHostFree(c->host_scalars);
+    delete c;
+    return QM_EHIP;
+  }
   *out = c;
   return QM_OK;
 }

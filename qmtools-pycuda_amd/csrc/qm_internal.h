@@ -177,6 +177,8 @@ int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol, hipStream_t s);
 void qm_fft_cache_free(FftPlanCache* c);
 void qm_gto_cache_free(void* c);
 void qm_reclaim_graphs(qm_ctx* ctx, bool wait);
+// device address of the allocation-free reduction's scratch, looked up once per GPU (reduce.hip)
+int qm_noalloc_setup(int device);
 
 // Brackets one kernel launch with events on its stream when profiling is on.
 struct ProfScope {

@@ -4,6 +4,8 @@
 // contiguous segment, pass 2 sums the block partials in a fixed tree order,
 // so results are bitwise reproducible (the reference uses FP32 atomicAdd
 // across blocks, kernel_utils.cu:97, whose order is not fixed).
+#include <mutex>
+
 #include "qm_internal.h"
 
 namespace {
@@ -115,8 +117,43 @@ void launch_pass1(hipStream_t s, int nb, const void* a, const void* b, int64_t n
 
 }  // namespace
 
-int qm_reduce_dev(qm_ctx* ctx, hipStream_t s, int op, int dtype, const void* a, const void* b, int64_t n,
-                  double* dev_result) {
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free reduction (qm_reduce_noalloc): the block partials
+// (kRedMaxBlocks doubles) and the device result, 16.6 KB.  Every GPU the library runs on
+// holds its own copy until the process ends.  In host code `scratch` is only a handle
+// for hipGetSymbolAddress; its device address, one per GPU, is looked up once when a
+// context is created on that GPU (qm_noalloc_setup) and used everywhere else.
+constexpr int kNoallocDoubles = kRedMaxBlocks + 32;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};   // device address of `scratch` on each GPU
+
+double* scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
+}  // namespace
+
+int qm_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return QM_ELIMIT;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return QM_OK;
+  void* p = nullptr;   // the caller has made `device` current
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) {
+    (void)hipGetLastError();
+    return QM_EHIP;
+  }
+  g_scratch[device] = (double*)p;
+  return QM_OK;
+}
+
+// Both passes with the block partials in `p` (kRedMaxBlocks doubles) and the result in dev_result.
+static int reduce_into(qm_ctx* ctx, hipStream_t s, int op, int dtype, const void* a, const void* b, int64_t n,
+                       double* p, double* dev_result) {
   QM_REQUIRE(ctx, a != nullptr && n >= 0, QM_EINVAL, "reduce: bad input");
   QM_REQUIRE(ctx, op == QM_RED_SUM || b != nullptr, QM_EINVAL, "reduce: second operand missing");
   QM_REQUIRE(ctx, dtype == QM_F64 || dtype == QM_F32, QM_EINVAL, "reduce: bad dtype");
@@ -125,10 +162,6 @@ int qm_reduce_dev(qm_ctx* ctx, hipStream_t s, int op, int dtype, const void* a, 
   if (nb < 1) nb = 1;
   if (nb > kRedMaxBlocks) nb = kRedMaxBlocks;
   int64_t seg = (n + nb - 1) / nb;
-  void* part = nullptr;
-  int rc = qm_ws(ctx, s, WS_RED, sizeof(double) * kRedMaxBlocks, &part);
-  if (rc) return rc;
-  double* p = (double*)part;
   if (dtype == QM_F64) {
     if (op == QM_RED_SUM) launch_pass1<double, QM_RED_SUM>(s, (int)nb, a, b, n, seg, p);
     else if (op == QM_RED_ABSDIFF) launch_pass1<double, QM_RED_ABSDIFF>(s, (int)nb, a, b, n, seg, p);
@@ -142,6 +175,14 @@ int qm_reduce_dev(qm_ctx* ctx, hipStream_t s, int op, int dtype, const void* a, 
   hipLaunchKernelGGL(k_reduce_pass2, dim3(1), dim3(kRedThreads), 0, s, (const double*)p, (int)nb, dev_result);
   QM_CHECK_LAUNCH(ctx);
   return QM_OK;
+}
+
+int qm_reduce_dev(qm_ctx* ctx, hipStream_t s, int op, int dtype, const void* a, const void* b, int64_t n,
+                  double* dev_result) {
+  void* part = nullptr;
+  int rc = qm_ws(ctx, s, WS_RED, sizeof(double) * kRedMaxBlocks, &part);
+  if (rc) return rc;
+  return reduce_into(ctx, s, op, dtype, a, b, n, (double*)part, dev_result);
 }
 
 int qm_scale_dev(qm_ctx* ctx, hipStream_t s, int dtype, void* a, int64_t n, const double* dev_num,
@@ -176,6 +217,26 @@ int qm_reduce(qm_ctx* ctx, void* stream, int op, int dtype, const void* a, const
   rc = qm_reduce_dev(ctx, s, op, dtype, a, b, n, (double*)dres);
   if (rc) return rc;
   QM_CHECK_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, dres, sizeof(double), hipMemcpyDeviceToHost, s));
+  QM_CHECK_HIP(ctx, hipStreamSynchronize(s));
+  *result = ctx->host_scalars[0];
+  return QM_OK;
+}
+
+int qm_reduce_noalloc(qm_ctx* ctx, void* stream, int op, int dtype, const void* a, const void* b, int64_t n,
+                      double* result) {
+  int rc = qm_enter(ctx);
+  if (rc) return rc;
+  QM_REQUIRE(ctx, result != nullptr, QM_EINVAL, "reduce_noalloc: null result");
+  QM_REQUIRE(ctx, n < ((int64_t)1 << 34), QM_ELIMIT,
+             "reduce_noalloc: %lld elements is beyond the 2^34 limit of the library's grids", (long long)n);
+  double* sc = scratch_of(ctx->device);
+  QM_REQUIRE(ctx, sc != nullptr, QM_EHIP, "reduce_noalloc: the scratch address of device %d was not found",
+             ctx->device);
+  hipStream_t s = (hipStream_t)stream;
+  // kernels and copies take the device address of the scratch, never its host-side symbol
+  rc = reduce_into(ctx, s, op, dtype, a, b, n, sc, sc + kRedMaxBlocks);
+  if (rc) return rc;
+  QM_CHECK_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, sc + kRedMaxBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
   QM_CHECK_HIP(ctx, hipStreamSynchronize(s));
   *result = ctx->host_scalars[0];
   return QM_OK;

@@ -63,6 +63,7 @@ SIGNATURES = {
     "qm_density": [_vp, _vp, _vp, _pg, _i, _i, _i, _vp, _i, _pd],
     "qm_basis_points": [_vp, _vp, _vp, _vp, _i64, _vp],
     "qm_reduce": [_vp, _vp, _i, _i, _vp, _vp, _i64, _pd],
+    "qm_reduce_noalloc": [_vp, _vp, _i, _i, _vp, _vp, _i64, _pd],
     "qm_scale": [_vp, _vp, _i, _vp, _i64, _d],
     "qm_seed": [_vp, _vp, _vp, _pg, _i, _vp],
     "qm_multipole_boundary": [_vp, _vp, _pg, _i, _vp, _vp, _pd],

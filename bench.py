@@ -164,8 +164,8 @@ def oracle_sample_check(w, nsample=4096, seed=9):
     return err, gpu_total, tot.value
 
 
-def cpu_baseline(w, budget_s=20.0):
-    """FP64 NumPy oracle on the host cores, on bounded samples of the same workload."""
+def cpu_baseline(w):
+    """FP64 NumPy oracle on the host cores, on bounded samples of the same workload, per component."""
     from oracle import density as od
     from oracle import fft as off
     from oracle import gto as og
@@ -175,7 +175,7 @@ def cpu_baseline(w, budget_s=20.0):
     sysm = make_system(w.mol.types, w.mol.coords, w.mol.dm, read_basis(os.path.join(PKG, "qmtools", "data",
                                                                                      "cc-pvdz.bin")))
     nx, ny, nz, h, o = w.grid.geometry()
-    # density: a slab of whole x-rows at the molecule's centre plane
+    # density: random voxels of the grid, scaled by the voxel count
     ns = 32768
     rng = numpy.random.default_rng(5)
     idx = rng.choice(w.N, size=ns, replace=False)
@@ -192,31 +192,95 @@ def cpu_baseline(w, budget_s=20.0):
     t_small = time.perf_counter() - t0
     M_big, M_small = (2 * nx) ** 3, (2 * m) ** 3
     t_fft = t_small * (M_big * math.log2(M_big)) / (M_small * math.log2(M_small))
-    # GTO: the pure-Python oracle on the first 8 atoms at 4 points, scaled by the primitive-pair count
-    # of the full molecule and by the number of points
-    def npairs(shells):
-        n = [len(a) for (_, _, a, _) in shells]
-        return sum(n[i] * n[j] for i in range(len(n)) for j in range(i + 1))
-    ka = 8
-    nbf_sub = sum(2 * L + 1 for (a, L, _, _) in sysm.shells if a < ka)
-    sub = make_system(w.mol.types[:ka], w.mol.coords[:ka], w.mol.dm[:nbf_sub, :nbf_sub],
-                      read_basis(os.path.join(PKG, "qmtools", "data", "cc-pvdz.bin")))
-    kp = 4
+    # GTO: the oracle's shell-pair loop over all 10^4 points (it is vectorised over points), for every
+    # 16th outer shell, scaled by the primitive pairs of the whole pair sum
+    npr = [len(a) for (_, _, a, _) in sysm.shells]
+    pair_cost = [npr[sa] * sum(npr[:sa + 1]) for sa in range(len(npr))]
+    sub = list(range(0, len(npr), 16))
     t0 = time.perf_counter()
-    og.potential(sub, w.pts_host[:kp])
-    t_gto = (time.perf_counter() - t0) * (npairs(sysm.shells) / npairs(sub.shells)) * w.npoints / kp
+    og.potential(sysm, w.pts_host, nuclei=False, shells_a=sub)
+    t_gto = (time.perf_counter() - t0) * sum(pair_cost) / sum(pair_cost[sa] for sa in sub)
     total = t_dens + t_fft + t_gto
     return {
         "value": w.N / total,
         "unit": "grid pts/s",
         "cores": cores,
         "kind": "port",
+        "seconds_per_step": {"density": t_dens, "fft_poisson": t_fft, "gto_points": t_gto},
         "sample": (f"NumPy FP64 oracle (OpenBLAS, {cores} threads): density on {ns} random voxels of 256^3 "
-                   f"extrapolated x{w.N // ns}; free-space FFT Poisson timed on 64^3 (128^3 padded) and scaled by "
-                   f"M log M to 512^3; GTO potential of the first {ka} atoms at {kp} points scaled by primitive pairs "
-                   f"and to {w.npoints} points; "
-                   f"seconds per step: density {t_dens:.1f}, FFT {t_fft:.1f}, GTO {t_gto:.1f} (extrapolated)"),
+                   f"scaled x{w.N // ns}; free-space FFT Poisson timed on 64^3 (128^3 padded) and scaled by "
+                   f"M log M to 512^3; GTO potential at all {w.npoints} points for every 16th outer shell "
+                   f"({len(sub)} of {len(npr)}) scaled by primitive pairs; seconds per step (extrapolated): "
+                   f"density {t_dens:.1f}, FFT {t_fft:.1f}, GTO {t_gto:.1f}"),
     }
+
+
+def cpu_small_workload():
+    """The 12-atom, 96^3 workload run in full by the NumPy oracle on the host: FP64 density, multipole
+    boundary, 2000 Jacobi sweeps (the oracle's sweep loop, as the GPU does them)."""
+    import qmtools
+    from oracle import density as od
+    from oracle import jacobi as oj
+    from oracle import seed as osd
+    from oracle.basisfile import read_basis
+    from oracle.system import make_system
+    from qmtools import synth
+    mol, grid = synth.molecule_12_grid_96(qmtools.BasisSet(os.path.join(PKG, "qmtools", "data", "cc-pvdz.bin")))
+    nx, ny, nz, h, o = grid.geometry()
+    sysm = make_system(mol.types, mol.coords, mol.dm, read_basis(os.path.join(PKG, "qmtools", "data",
+                                                                                     "cc-pvdz.bin")))
+    t0 = time.perf_counter()
+    q, _ = od.density_grid(sysm, nx, ny, nz, h, o)
+    t1 = time.perf_counter()
+    rho = osd.seed_grid(mol.types, mol.coords, nx, ny, nz, h, o) - q
+    v = oj.jacobi(rho, nx, ny, nz, h, 2000, bvals=oj.multipole_boundary(rho, nx, ny, nz, h, o))
+    t2 = time.perf_counter()
+    npts = nx * ny * nz
+    return {"workload": "12-atom cc-pVDZ (123 bf), 96^3: density + multipole boundary + 2000 Jacobi sweeps",
+            "cores": len(os.sched_getaffinity(0)), "kind": "port", "seconds": {"density": t1 - t0, "jacobi": t2 - t1},
+            "grid_pts_per_s": npts / (t2 - t0), "v_checksum": float(numpy.sum(v))}
+
+
+def gpu_small_workload():
+    """The same 12-atom, 96^3 workload on the GPU through the drop-in API (FP64 and mixed)."""
+    import torch
+    import qmtools
+    from qmtools import synth
+    mol, grid = synth.molecule_12_grid_96(qmtools.BasisSet(os.path.join(PKG, "qmtools", "data", "cc-pvdz.bin")))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        for rep in range(2):   # the first run uploads the molecule and builds the caches
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            q = qmtools.QMTools.Compute_density(grid, mol, precision=prec, copyBack=False)
+            v = qmtools.QMTools.Compute_hartree_iter(q, mol, tolerance=None, max_sweeps=2000, boundary="multipole",
+                                                     precision=prec, copyBack=False)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        out[prec] = {"seconds": dt, "grid_pts_per_s": int(grid.npts) / dt, "sweeps": int(v.sweeps)}
+    out["workload"] = "12-atom cc-pVDZ (123 bf), 96^3: density + multipole boundary + 2000 Jacobi sweeps"
+    return out
+
+
+def cold_request(basis):
+    """First request of a molecule never seen before, on a grid shape never seen before: molecule upload,
+    GTO distribution table and Green's-function transform included, against the same call repeated."""
+    import torch
+    import qmtools
+    from qmtools import synth
+    mol = synth.synthetic_molecule(40, 913, basis)
+    grid = synth.centred_grid(mol, 152, step_bohr=0.13, pad_ang=3.0)
+    pts = synth.uniform_points(grid, 10000, 4)
+    out = {"workload": f"40-atom molecule ({mol.nbf} bf), 152^3, mixed density + free-space FFT + GTO at 1e4 points"}
+    for rep in ("cold", "warm"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        q = qmtools.QMTools.Compute_density(grid, mol, precision="mixed", copyBack=False)
+        qmtools.QMTools.Compute_hartree_fft(q, mol, mode="freespace", precision="mixed")
+        qmtools.QMTools.Compute_hartree_points(mol, pts)
+        torch.cuda.synchronize()
+        out[rep + "_s"] = time.perf_counter() - t0
+    return out
 
 
 def jacobi_section(w, sweeps=200):
@@ -307,20 +371,31 @@ def batched_section(basis, count, sweeps, seed=41):
     return out
 
 
-def served_section(basis, nreq, rate, sweeps, seed=77, grid_n=(64, 96, 128)):
-    """Config 5: Poisson arrivals at `rate` req/s into the continuous-batching server (mixed path)."""
-    from qmtools import serve, synth
-    mols, grids = synth.molecule_batch(nreq, (5, 120), grid_n, (0.1, 0.15), seed, basis)
-    srv = serve.Server(nlanes=2, precision="mixed", max_batch=8, sweeps=sweeps, hbm_budget=96 << 30,
-                       to_host=False).start()
-    srv.submit(mols[0], grids[0]).wait(600)                        # warm-up
+def serve_pool(basis, count=250, seed=77, grid_n=(64, 96, 128, 160, 192, 224, 256)):
+    """Seeded molecules of 5-120 atoms on 64^3-256^3 grids (0.1-0.15 bohr) that requests are drawn from."""
+    from qmtools import synth
+    return list(zip(*synth.molecule_batch(count, (5, 120), grid_n, (0.1, 0.15), seed, basis)))
+
+
+def served_section(pool, nreq, rate, sweeps, precision="mixed", poisson="fft", seed=5):
+    """Poisson arrivals at `rate` req/s into the continuous-batching server; each request draws a
+    molecule and its grid from `pool`.  Results are dropped on completion (throughput and latency only)."""
+    from qmtools import serve
+    rng = numpy.random.default_rng(seed)
+    items = [pool[int(i)] for i in rng.integers(0, len(pool), size=nreq)]
+    srv = serve.Server(nlanes=2, precision=precision, max_batch=8, sweeps=sweeps, hbm_budget=160 << 30,
+                       poisson=poisson, results="drop").start()
+    for m, g in pool[:2]:                                            # warm-up
+        srv.submit(m, g).wait(600)
     srv.completed.clear()
-    reqs = serve.poisson_arrivals(srv, list(zip(mols, grids)), rate, seed)
+    t0 = time.perf_counter()
+    reqs = serve.poisson_arrivals(srv, items, rate, seed)
     for r in reqs:
         r.wait(1200)
     srv.stop()
     st = srv.stats()
-    st.update({"rate_req_per_s": rate, "sweeps": sweeps})
+    st.update({"rate_req_per_s": rate, "precision": precision, "poisson": poisson,
+               "sweeps": sweeps if poisson == "jacobi" else None, "wall_s": time.perf_counter() - t0})
     return st
 
 
@@ -334,12 +409,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the comparison run of the other precision")
-    ap.add_argument("--batch", type=int, default=16, help="molecules in the batched section (0: skip)")
-    ap.add_argument("--serve", type=int, default=48, help="requests in the served section (0: skip)")
-    ap.add_argument("--rate", type=str, default="100",
-                    help="Poisson arrival rate(s) of the served section, comma-separated (req/s)")
-    ap.add_argument("--serve-grids", type=str, default="64,96,128",
-                    help="cubic grid sizes drawn for served requests, comma-separated")
+    ap.add_argument("--batch", type=int, default=256, help="molecules in the batched section (0: skip)")
+    ap.add_argument("--serve", type=int, default=1000, help="requests per rate in the served section (0: skip)")
+    ap.add_argument("--rate", type=str, default="50,100,200,500",
+                    help="Poisson arrival rates of the served section (FFT Poisson), comma-separated (req/s)")
+    ap.add_argument("--serve-jacobi", type=int, default=100,
+                    help="requests of the served Jacobi runs (2000 sweeps, 10 req/s, mixed and FP64; 0: skip)")
+    ap.add_argument("--no-cpu-full", action="store_true",
+                    help="skip the full host run of the 12-atom, 96^3 workload")
     ap.add_argument("--sweeps", type=int, default=2000, help="Jacobi sweeps in the batched/served sections")
     ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 Jacobi and diffusion V_ne sections")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -440,10 +517,18 @@ def main():
         err, gsum, raw = oracle_sample_check(w)
         check = {"density_sample_rel_err_vs_numpy_fp64": err, "voxels": 4096, "normalised_sum": gsum,
                  "raw_sum": raw}
-    cpu = None
+    cpu = cpu_small = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cpu = cpu_baseline(w)
+        log(f"cpu baseline: {cpu['seconds_per_step']}")
+        if not args.no_cpu_full:
+            cpu_small = cpu_small_workload()
+            log(f"cpu 12 atoms 96^3: {cpu_small}")
+    gpu_small = gpu_small_workload()
+    log(f"gpu 12 atoms 96^3: {gpu_small}")
+    cold = cold_request(w.basis)
+    log(f"cold request: {cold}")
 
     jac = vne = None
     if not args.no_extra:
@@ -456,15 +541,24 @@ def main():
         batched = batched_section(w.basis, args.batch, args.sweeps)
         for k in ("jacobi", "fft"):
             log(f"batched ({k}): {batched[k]['speedup']:.2f}x plain FP64, errors {batched[k]['max_rel_err_vs_fp64']}")
-    if args.serve > 0:
-        gn = tuple(int(x) for x in args.serve_grids.split(","))
-        served = []
-        for rate in (float(x) for x in args.rate.split(",")):
-            st = served_section(w.basis, args.serve, rate, args.sweeps, grid_n=gn)
-            log(f"served: {st}")
-            served.append(st)
-        if len(served) == 1:
-            served = served[0]
+    if args.serve > 0 or args.serve_jacobi > 0:
+        pool = serve_pool(w.basis)
+        served = {"fft_mixed": [], "fft_plain_fp64": None, "jacobi": None,
+                  "requests": "Poisson arrivals; molecules of 5-120 atoms on 64^3-256^3 grids drawn from a "
+                              "seeded pool of 250"}
+        if args.serve > 0:
+            for rate in (float(x) for x in args.rate.split(",")):
+                st = served_section(pool, args.serve, rate, args.sweeps)
+                log(f"served fft mixed: {st}")
+                served["fft_mixed"].append(st)
+            st = served_section(pool, args.serve, 100.0, args.sweeps, precision="fp64")
+            log(f"served fft fp64: {st}")
+            served["fft_plain_fp64"] = st
+        if args.serve_jacobi > 0:
+            served["jacobi"] = [served_section(pool, args.serve_jacobi, 10.0, args.sweeps, precision=p,
+                                               poisson="jacobi") for p in ("mixed", "fp64")]
+            log(f"served jacobi: {served['jacobi']}")
+        del pool
 
     value = w.N / (ms_step / 1000.0)
     out = {
@@ -506,6 +600,9 @@ def main():
         "served": served,
         "oracle_check": check,
         "autotune": tune,
+        "small_workload_gpu": gpu_small,
+        "small_workload_cpu": cpu_small,
+        "cold_request": cold,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

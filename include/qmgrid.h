@@ -73,6 +73,12 @@ typedef struct {
 int qm_ctx_create(int device, qm_ctx** out);
 int qm_ctx_destroy(qm_ctx* ctx);
 const char* qm_last_error(const qm_ctx* ctx);
+
+/* Device bytes a context holds between calls: its grow-only workspaces plus the FFT twiddle
+ * tables and cached Green's-function transforms (at most 4 GiB of those per context, least
+ * recently used released first).  No reference counterpart; lets a request loop charge each
+ * lane's resident memory against its device-memory budget. */
+int qm_ctx_memory(const qm_ctx* ctx, int64_t* bytes);
 /* ABI version: major*10000 + minor*100 + patch */
 int qm_version(void);
 

@@ -114,6 +114,14 @@ int qm_ctx_destroy(qm_ctx* ctx) {
 
 const char* qm_last_error(const qm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int qm_ctx_memory(const qm_ctx* ctx, int64_t* bytes) {
+  if (!ctx || !bytes) return QM_EINVAL;
+  size_t b = qm_fft_cache_bytes(ctx->fft);
+  for (const auto& w : ctx->ws) b += w.bytes;
+  *bytes = (int64_t)b;
+  return QM_OK;
+}
+
 static void clear_records(qm_ctx* ctx) {
   for (auto& r : ctx->recs) {
     (void)hipEventDestroy(r.a);

@@ -445,21 +445,34 @@ std::vector<int> factorize(int n) {
 
 }  // namespace
 
+// Transforms of the free-space Green's function, one per (grid shape, step, precision), are
+// kept up to kGreenCacheBytes per context; beyond that the least recently used are released
+// (stream-ordered, from the stream-ordered allocator).
+constexpr size_t kGreenCacheBytes = (size_t)4 << 30;
+
 struct FftPlanCache {
   std::map<std::pair<int, int>, void*> twiddles;            // (n, bytes per real) -> device table
+  size_t twiddle_bytes = 0;
   struct Green {
     int n0, n1, n2;
     double h;
     int rbytes;
     void* gh;
+    size_t bytes;
+    uint64_t used;   // last use (cache clock)
   };
   std::vector<Green> greens;
+  size_t green_bytes = 0;
+  uint64_t clock = 0;
 };
+
+size_t qm_fft_cache_bytes(const FftPlanCache* c) { return c ? c->twiddle_bytes + c->green_bytes : 0; }
 
 void qm_fft_cache_free(FftPlanCache* c) {
   if (!c) return;
   for (auto& kv : c->twiddles) (void)hipFree(kv.second);
-  for (auto& g : c->greens) (void)hipFree(g.gh);
+  for (auto& g : c->greens) (void)hipFreeAsync(g.gh, nullptr);
+  (void)hipStreamSynchronize(nullptr);
   delete c;
 }
 
@@ -484,6 +497,7 @@ int twiddle_table(qm_ctx* ctx, int n, const Cx<R>** out) {
   QM_CHECK_HIP(ctx, hipMalloc(&d, sizeof(Cx<R>) * n));
   QM_CHECK_HIP(ctx, hipMemcpy(d, h.data(), sizeof(Cx<R>) * n, hipMemcpyHostToDevice));
   ctx->fft->twiddles[key] = d;
+  ctx->fft->twiddle_bytes += sizeof(Cx<R>) * n;
   *out = (const Cx<R>*)d;
   return QM_OK;
 }
@@ -558,8 +572,10 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
 template <typename R>
 int green_hat(qm_ctx* ctx, hipStream_t s, const qm_geom* g, Cx<R>* work, const R** out) {
   if (!ctx->fft) ctx->fft = new FftPlanCache();
-  for (auto& gr : ctx->fft->greens)
+  FftPlanCache& fc = *ctx->fft;
+  for (auto& gr : fc.greens)
     if (gr.n0 == g->nx && gr.n1 == g->ny && gr.n2 == g->nz && gr.h == g->step && gr.rbytes == (int)sizeof(R)) {
+      gr.used = ++fc.clock;
       *out = (const R*)gr.gh;
       return QM_OK;
     }
@@ -573,13 +589,24 @@ int green_hat(qm_ctx* ctx, hipStream_t s, const qm_geom* g, Cx<R>* work, const R
   if ((rc = fft_pass<R, false>(ctx, s, work, m, 2, m[0], m[1]))) return rc;
   if ((rc = fft_pass<R, false>(ctx, s, work, m, 1, m[0], m[2]))) return rc;
   if ((rc = fft_pass<R, false>(ctx, s, work, m, 0, m[1], m[2]))) return rc;
+  // make room: release the least recently used transforms (work queued earlier on s still reads them)
+  const size_t bytes = sizeof(R) * (size_t)M;
+  while (!fc.greens.empty() && fc.green_bytes + bytes > kGreenCacheBytes) {
+    size_t lru = 0;
+    for (size_t i = 1; i < fc.greens.size(); ++i)
+      if (fc.greens[i].used < fc.greens[lru].used) lru = i;
+    QM_CHECK_HIP(ctx, hipFreeAsync(fc.greens[lru].gh, s));
+    fc.green_bytes -= fc.greens[lru].bytes;
+    fc.greens.erase(fc.greens.begin() + lru);
+  }
   void* gh = nullptr;
-  QM_CHECK_HIP(ctx, hipMalloc(&gh, sizeof(R) * M));
+  QM_CHECK_HIP(ctx, hipMallocAsync(&gh, bytes, s));
   // V = -(h^3 / 4 pi) IFFT(rhs^ g^) / M   (rhs = -rho/eps0, g = 1/r)
   const double scale = -(h_ang * h_ang * h_ang) / (4.0 * M_PI) / (double)M;
   hipLaunchKernelGGL(k_green_store<R>, dim3(blocks), dim3(256), 0, s, (const Cx<R>*)work, (R*)gh, M, scale);
   QM_CHECK_LAUNCH(ctx);
-  ctx->fft->greens.push_back({g->nx, g->ny, g->nz, g->step, (int)sizeof(R), gh});
+  fc.greens.push_back({g->nx, g->ny, g->nz, g->step, (int)sizeof(R), gh, bytes, ++fc.clock});
+  fc.green_bytes += bytes;
   *out = (const R*)gh;
   return QM_OK;
 }

@@ -175,6 +175,7 @@ int qm_scale_dev(qm_ctx* ctx, hipStream_t s, int dtype, void* a, int64_t n, cons
 // density-matrix device forms (density.hip)
 int qm_density_upload_P(qm_ctx* ctx, qm_mol* mol, hipStream_t s);
 void qm_fft_cache_free(FftPlanCache* c);
+size_t qm_fft_cache_bytes(const FftPlanCache* c);
 void qm_gto_cache_free(void* c);
 void qm_reclaim_graphs(qm_ctx* ctx, bool wait);
 // device address of the allocation-free reduction's scratch, looked up once per GPU (reduce.hip)

@@ -54,6 +54,7 @@ SIGNATURES = {
     "qm_ctx_create": [_i, _c.POINTER(_vp)],
     "qm_ctx_destroy": [_vp],
     "qm_last_error": [_vp],
+    "qm_ctx_memory": [_vp, _c.POINTER(_i64)],
     "qm_mol_create": [_vp, _i, _pi, _pd, _i, _pi, _pi, _pi, _pi, _i, _pd, _pd, _c.POINTER(_vp)],
     "qm_mol_set_dm": [_vp, _vp, _vp, _pd],
     "qm_mol_set_coords": [_vp, _vp, _vp, _pd],

@@ -4,9 +4,12 @@ Requests (molecule, grid) arrive at any time (`Server.submit`).  One loop
 thread owns every GPU call:
 
 * admission: pending requests join the next batch while the batch's device
-  memory estimate plus the memory of the batches already on the GPU stays
-  within `hbm_budget` (bytes) and the batch holds at most `max_batch`
-  requests; a request larger than the whole budget is refused;
+  memory estimate plus the memory of the batches already on the GPU plus
+  what every lane's native context keeps resident (its workspaces and FFT
+  caches, measured after each batch, or the batch's own workspace estimate
+  when that is larger) stays within `hbm_budget` (bytes), and the batch holds
+  at most `max_batch` requests; a request larger than the whole budget is
+  refused;
 * execution: each batch runs on a free lane (its own native context and
   stream), so a batch admitted while others are running overlaps with them
   on the device;
@@ -64,36 +67,78 @@ class Request:
         return None if self.t_done is None else self.t_done - self.t_submit
 
     def cancel(self):
-        """Drop the request if it has not been admitted yet; returns True if it was dropped."""
-        self.cancelled = True
-        return self.t_admit is None
+        """Drop the request if it has not been admitted yet; returns True if it was dropped.
+
+        Admission marks a request under the server's lock, and so does this test, so a request
+        is either dropped here or runs, never both."""
+        lock = getattr(self, "_lock", None)
+        if lock is None:
+            self.cancelled = True
+            return self.t_admit is None
+        with lock:
+            if self._admitted:
+                return False
+            self.cancelled = True
+            return True
+
+    _admitted = False
 
 
-def memory_estimate(req, precision="mixed"):
-    """Device bytes of one request in a batch (grids of the density and Hartree steps, P)."""
+def memory_estimate(req, precision="mixed", poisson="jacobi"):
+    """Device bytes of one request in a batch: its density and potential grids and the per-item
+    buffers of the Hartree step (seed, rho, held boundary and, in FP64, the sweep partner), plus P."""
     n = req.npts
-    per_point = 8 * 5 if precision == "fp64" else 4 + 8 * 4 + 4 * 3
+    if poisson == "fft":
+        per_point = 8 + 8 if precision == "fp64" else 4 + 4
+    else:
+        per_point = 8 * 5 if precision == "fp64" else 4 + 8 * 4 + 4 * 3
     nbf = int(req.mol.nbf)
     return n * per_point + 16 * nbf * nbf
 
 
-class GpuExecutor:
-    """Runs batches with `qmtools.batch.run` on per-lane contexts and streams."""
+_PHI_BUDGET = 4 << 30          # the native density keeps at most this much basis values per round
+_GREEN_BUDGET = 4 << 30        # and at most this much of cached Green's-function transforms
 
-    def __init__(self, nlanes=2, precision="mixed", sweeps=2000, boundary="multipole", to_host=True):
+
+def workspace_estimate(reqs, precision="mixed", poisson="jacobi"):
+    """Device bytes a lane's native context grows to while it runs this batch: basis values of one
+    round, the FP32 Jacobi correction buffers or the FFT work buffer and Green's-function cache."""
+    elt = 8 if precision == "fp64" else 4
+    npts = sum(r.npts for r in reqs)
+    biggest = max((r.npts for r in reqs), default=0)
+    nbf_pad = max(((int(r.mol.nbf) + 63) // 64 * 64 for r in reqs), default=0)
+    phi = min(_PHI_BUDGET, npts * nbf_pad * elt)
+    if poisson == "fft":
+        cplx = 16 if precision == "fp64" else 8
+        hartree = 8 * biggest * cplx + min(_GREEN_BUDGET, sum(8 * r.npts * elt for r in reqs))
+    else:
+        hartree = 0 if precision == "fp64" else 12 * npts
+    return phi + hartree
+
+
+class GpuExecutor:
+    """Runs batches with `qmtools.batch.run` on per-lane contexts and streams.
+
+    results: "host" copies density and potential to the host, "device" keeps the device
+    tensors on the request, "drop" releases them (throughput runs)."""
+
+    def __init__(self, nlanes=2, precision="mixed", sweeps=2000, boundary="multipole", to_host=True,
+                 poisson="jacobi", results=None):
         from . import batch
         self._batch = batch
+        self.nlanes = nlanes
         self.lanes = [batch.Lane() for _ in range(nlanes)]
         self.precision = precision
         self.sweeps = sweeps
         self.boundary = boundary
-        self.to_host = to_host
+        self.poisson = poisson
+        self.results = results or ("host" if to_host else "device")
 
     def launch(self, lane_id, reqs):
         lane = self.lanes[lane_id]
         torch = __import__("torch")
         qs, vs = self._batch.run([r.mol for r in reqs], [r.grid for r in reqs], self.precision, self.sweeps,
-                                 self.boundary, lane)
+                                 self.boundary, lane, poisson=self.poisson)
         ev = torch.cuda.Event()
         ev.record(lane.stream)
         return (ev, qs, vs)
@@ -105,11 +150,19 @@ class GpuExecutor:
         ev, qs, vs = handle
         ev.synchronize()
         for r, q, v in zip(reqs, qs, vs):
-            if self.to_host:
+            if self.results == "host":
                 r.density = q.cpu().numpy()
                 r.potential = v.cpu().numpy()
-            else:
+            elif self.results == "device":
                 r.density, r.potential = q, v
+
+    def resident_bytes(self, lane_id):
+        """What the lane's native context keeps on the device between batches."""
+        import ctypes
+        from . import _native as nat
+        b = ctypes.c_int64(0)
+        nat.check(nat.library().qm_ctx_memory(self.lanes[lane_id].ctx, ctypes.byref(b)), self.lanes[lane_id].ctx)
+        return int(b.value)
 
     def close(self):
         for lane in self.lanes:
@@ -118,10 +171,14 @@ class GpuExecutor:
 
 class Server:
     def __init__(self, executor=None, nlanes=2, precision="mixed", hbm_budget=64 << 30, max_batch=16,
-                 sweeps=2000, boundary="multipole", to_host=True, idle_sleep=2e-4):
+                 sweeps=2000, boundary="multipole", to_host=True, idle_sleep=2e-4, poisson="jacobi",
+                 results=None):
         self.precision = precision
-        self.executor = executor or GpuExecutor(nlanes, precision, sweeps, boundary, to_host)
+        self.poisson = poisson
+        self.executor = executor or GpuExecutor(nlanes, precision, sweeps, boundary, to_host, poisson, results)
         self.nlanes = nlanes if executor is None else getattr(executor, "nlanes", nlanes)
+        self._resident = [0] * self.nlanes   # measured resident bytes of each lane's context
+        self._reserved = [0] * self.nlanes   # what each lane is charged: max(resident, running batch's workspace)
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.idle_sleep = idle_sleep
@@ -138,7 +195,9 @@ class Server:
     # -- client side ---------------------------------------------------------
     def submit(self, mol, grid, tag=None):
         req = Request(mol, grid, tag)
-        if memory_estimate(req, self.precision) > self.hbm_budget:
+        req._lock = self._lock
+        if memory_estimate(req, self.precision, self.poisson) + \
+                workspace_estimate([req], self.precision, self.poisson) > self.hbm_budget:
             req.error = MemoryError(f"request {req.rid} needs more than the HBM budget of {self.hbm_budget} B")
             req.t_done = time.perf_counter()
             req._done.set()
@@ -184,9 +243,16 @@ class Server:
                 self.completed.append(r)
             del self._inflight[lane]
             self._used -= nbytes
+            measure = getattr(self.executor, "resident_bytes", None)
+            if measure is not None:
+                self._resident[lane] = max(self._resident[lane], measure(lane))
+            self._reserved[lane] = self._resident[lane]
 
     def _admit(self, lane):
+        """Requests for a batch on `lane`; returns (batch, request bytes, lane charge)."""
         batch, nbytes = [], 0
+        others = sum(self._reserved) - self._reserved[lane]
+        charge = self._resident[lane]
         with self._lock:
             while self._pending and len(batch) < self.max_batch:
                 r = self._pending[0]
@@ -196,13 +262,16 @@ class Server:
                     r.t_done = time.perf_counter()
                     r._done.set()
                     continue
-                need = memory_estimate(r, self.precision)
-                if self._used + nbytes + need > self.hbm_budget:
+                need = memory_estimate(r, self.precision, self.poisson)
+                lane_charge = max(self._resident[lane], workspace_estimate(batch + [r], self.precision, self.poisson))
+                if self._used + nbytes + need + others + lane_charge > self.hbm_budget:
                     break
                 self._pending.popleft()
+                r._admitted = True   # under the lock: cancel() can no longer drop it
                 batch.append(r)
                 nbytes += need
-        return batch, nbytes
+                charge = lane_charge
+        return batch, nbytes, charge
 
     def _loop(self):
         while not self._stop:
@@ -211,9 +280,10 @@ class Server:
             for lane in range(self.nlanes):
                 if lane in self._inflight:
                     continue
-                batch, nbytes = self._admit(lane)
+                batch, nbytes, charge = self._admit(lane)
                 if not batch:
                     break
+                self._reserved[lane] = charge
                 now = time.perf_counter()
                 for r in batch:
                     r.t_admit = now

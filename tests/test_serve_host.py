@@ -7,7 +7,7 @@ import numpy
 import pytest
 
 from qmtools import Grid, Molecule
-from qmtools.serve import Server, memory_estimate
+from qmtools.serve import Server, memory_estimate, workspace_estimate
 
 
 class FakeExecutor:
@@ -51,7 +51,8 @@ def _mol(basis):
 def test_admission_budget_and_overlap(basis):
     mol = _mol(basis)
     grid = Grid((1, 32, 32, 32), 0.2)
-    per = memory_estimate(type("R", (), {"npts": 32 ** 3, "mol": mol})())
+    fake = type("R", (), {"npts": 32 ** 3, "mol": mol})()
+    per = memory_estimate(fake) + workspace_estimate([fake])   # a request and the workspace it adds
     ex = FakeExecutor(nlanes=2, run_s=0.03)
     srv = Server(executor=ex, hbm_budget=3 * per, max_batch=2).start()
     reqs = [srv.submit(mol, grid) for _ in range(10)]
@@ -101,3 +102,45 @@ def test_cancel_before_admission(basis):
         second.wait(5)
     srv.stop()
     assert all(second.rid not in ids for _, ids in ex.launched)
+
+
+def test_lane_workspace_is_charged(basis):
+    """What a lane's context keeps resident after a batch counts against the budget: with one lane
+    holding a large workspace, a second batch that would fit on its own is not admitted beside it."""
+    mol = _mol(basis)
+    grid = Grid((1, 16, 16, 16), 0.2)
+    fake = type("R", (), {"npts": 16 ** 3, "mol": mol})()
+    per = memory_estimate(fake) + workspace_estimate([fake])
+
+    class Resident(FakeExecutor):
+        def resident_bytes(self, lane):
+            return 10 * per if lane == 0 else 0
+
+    ex = Resident(nlanes=2, run_s=0.02)
+    srv = Server(executor=ex, hbm_budget=11 * per, max_batch=1).start()
+    first = srv.submit(mol, grid)
+    first.wait(10)
+    time.sleep(0.05)                      # retired: lane 0 now holds 10 per resident
+    reqs = [srv.submit(mol, grid) for _ in range(6)]
+    for r in reqs:
+        r.wait(10)
+    srv.stop()
+    assert all(r.error is None for r in reqs)
+    # 10 resident + one request fits in 11; a second request beside it does not
+    assert srv.max_concurrent == 1
+
+
+def test_cancel_after_admission_keeps_result(basis):
+    """A request admitted into a batch cannot be dropped any more: cancel() says so and it completes."""
+    mol = _mol(basis)
+    grid = Grid((1, 16, 16, 16), 0.2)
+    ex = FakeExecutor(nlanes=1, run_s=0.2)
+    srv = Server(executor=ex, hbm_budget=10 ** 9, max_batch=1).start()
+    r = srv.submit(mol, grid)
+    t0 = time.perf_counter()
+    while not r._admitted and time.perf_counter() - t0 < 5:
+        time.sleep(1e-3)
+    assert r.cancel() is False
+    r.wait(10)
+    srv.stop()
+    assert r.error is None and r.density is not None

@@ -279,7 +279,9 @@ enum {
                                written pre-split, staged through registers, 3 = pre-split, fetched by LDS DMA
                                (default); 4 = basis values evaluated inside the contraction kernel, 128 points
                                x 320 columns per workgroup (no basis-value buffer; slower on the 60-atom
-                               workload); 64 or 128 = points per workgroup of the all-columns kernel */
+                               workload); 5 = pre-split, DMA, 256 points x 320 columns per workgroup and
+                               128 x 80 per wave (slower); 64 or 128 = points per workgroup of the
+                               all-columns kernel */
   QM_TUNE_F64_VARIANT = 1,  /* FP64 contraction: 0 = all columns per workgroup (wave-cyclic tiles), 1 = per-wave
                                loads of the dense P, no barriers, 2 = 128 x 128 tiles with per-block partial sums,
                                3 = 128 x 64 tiles with diagonal skipping, 4 = 128 x 128 tiles fed by LDS DMA with

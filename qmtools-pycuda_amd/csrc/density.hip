@@ -1570,6 +1570,168 @@ __global__ __launch_bounds__(512) void k_density_fused(MolView m, const float2* 
   fused_tile<T16>(rt, cg, tab, m, g, v0, nvox, nsub, npts, p_hi, p_lo, np, nbf, ncg, S, part, ldp);
 }
 
+// ---------------------------------------------------------------------------
+// Wide split contraction (split variant 5): basis values written pre-split by
+// k_phi_split, a workgroup of 8 waves owns 256 points and one column group of
+// 320 columns (right-aligned like the fused kernel's groups), and every wave
+// owns 128 points x 5 tiles of 16 columns: wave 4 h + w takes rows 128 h ..
+// 128 h + 127 and the tiles w, w + 4, .., w + 16 of the group (dealt
+// round-robin so the triangle leaves the waves evenly), 160 FP32 accumulators
+// each, two waves per SIMD.  Per stage of 32 k a wave reads 16 KB of basis-
+// value fragments and 10 KB of M fragments from LDS for 120 MFMAs (0.22 KB per
+// MFMA, against 0.5 KB for the 32 x 64 wave tiles of variant 3).  Both operands
+// come by LDS DMA (global_load_lds_dwordx4) into a two-stage ring: per stage
+// 32 KB of basis values (hi, lo) and 40 KB of M; the loads of stage st + 1 are
+// issued right after the barrier that opens stage st.  MFMA order, products
+// and row-dot are those of the fused kernel's consumers.  LDS: 144 KB + 8 KB.
+//
+// Measured on the 60-atom, 256^3 workload: bitwise the results of variant 3, but 30.4 ms of
+// contraction against 17.7 ms.  Counters (rocprofv3, same MFMA busy cycles in both): it waits
+// on LDS instruction issue 4.5x less than variant 3, but with one workgroup (two waves) per
+// SIMD pair the barrier and the fragment loads at the start of each stage leave the MFMA pipes
+// idle (35% busy against 52% for the four waves per SIMD of variant 3).  Kept as a variant.
+constexpr int kWP = 256;   // points per workgroup
+
+inline int wide_groups(int nbf) { return fused_groups(nbf); }
+
+template <typename T16>
+__device__ __forceinline__ void wide_tile(int rt, int cg, const T16* __restrict__ a_hi, int ld,
+                                          const T16* __restrict__ p_hi, const T16* __restrict__ p_lo, int np,
+                                          int nbf, int ncg, double* __restrict__ part, int ldp) {
+  typedef typename Vec8<T16>::type V8;
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  __shared__ u32x4 phs[2][2][4][kWP];   // basis values: [slot][hi, lo][k group][point]
+  __shared__ u32x4 mrs[2][2][4][kFC];   // M: [slot][hi, lo][k group][column]
+  __shared__ double red[2][4][128];
+  const T16* __restrict__ a_lo = a_hi + (int64_t)np * ld;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int hr = wave >> 2, w = wave & 3;   // row half, column slice
+  const int ne = (nbf + 31) / 32 * 32;
+  const int c0 = ne - (ncg - cg) * kFC;
+  const int nst = (c0 + kFC) / 32;
+  const int row0 = rt * kWP;
+  const int li = lane & 15, lk = lane >> 4;
+  // per stage: 32 DMA instructions of basis values (plane, k group, 64-point chunk) and 40 of M
+  // (plane, k group, 64-column chunk); wave v issues instructions v, v + 8, .., < 72
+  auto issue = [&](int st) __attribute__((always_inline)) {
+    const int slot = st & 1;
+    for (int i = wave; i < 72; i += 8) {
+      if (i < 32) {
+        const int pl = i >> 4, kg = (i >> 2) & 3, ch = i & 3;
+        const T16* src = pl ? a_lo : a_hi;
+        lds_dma16(src + ((int64_t)(4 * st + kg) * ld + row0 + 64 * ch + lane) * 8, &phs[slot][pl][kg][64 * ch]);
+      } else {
+        const int j = i - 32, pl = j / 20, kg = (j % 20) / 5, ch = j % 5;
+        const T16* src = pl ? p_lo : p_hi;
+        const int col = max(c0 + 64 * ch + lane, 0);   // columns below 0 belong to skipped tiles
+        lds_dma16(src + ((int64_t)(4 * st + kg) * np + col) * 8, &mrs[slot][pl][kg][64 * ch]);
+      }
+    }
+  };
+  f4 acc[8][5];
+  double pr[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    pr[r] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc[r][c] = f4{0, 0, 0, 0};
+  }
+  const int cpad = min(cdiv_floor0(-c0 - 16 * w, 64), 5);
+  auto first_live = [&](int st) { return min(max(cdiv_floor0(32 * st - c0 - 15 - 16 * w, 64), cpad), 5); };
+  const int rbase = 128 * hr;
+  auto body = [&](int st, auto first) __attribute__((always_inline)) {
+    constexpr int CS = decltype(first)::value;
+    const int slot = st & 1;
+    const int nfirst = c0 + 16 * (w + 4 * CS);
+    const bool done = nfirst < 32 * st + 32;
+    const int dc = (nfirst - 32 * st) >> 4;
+    V8 mh[5], ml[5];
+#pragma unroll
+    for (int c = CS; c < 5; ++c) {
+      mh[c] = __builtin_bit_cast(V8, mrs[slot][0][lk][16 * (w + 4 * c) + li]);
+      ml[c] = __builtin_bit_cast(V8, mrs[slot][1][lk][16 * (w + 4 * c) + li]);
+    }
+    V8 ah = __builtin_bit_cast(V8, phs[slot][0][lk][rbase + li]);
+    V8 al = __builtin_bit_cast(V8, phs[slot][1][lk][rbase + li]);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      V8 ahn = ah, aln = al;
+      if (r + 1 < 8) {
+        ahn = __builtin_bit_cast(V8, phs[slot][0][lk][rbase + 16 * (r + 1) + li]);
+        aln = __builtin_bit_cast(V8, phs[slot][1][lk][rbase + 16 * (r + 1) + li]);
+      }
+#pragma unroll
+      for (int c = CS; c < 5; ++c) {   // transposed tiles: M first
+        acc[r][c] = Vec8<T16>::mma(mh[c], al, acc[r][c]);
+        acc[r][c] = Vec8<T16>::mma(ml[c], ah, acc[r][c]);
+        acc[r][c] = Vec8<T16>::mma(mh[c], ah, acc[r][c]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      ah = ahn;
+      al = aln;
+    }
+    if (done) {
+      typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+      typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+      typedef typename std::conditional<std::is_same<T16, _Float16>::value, h4, b4>::type V4;
+      const int kg = 2 * dc + (lk >> 1);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const V4 h = __builtin_bit_cast(V4, ((const uint2*)&phs[slot][0][kg][rbase + 16 * r + li])[lk & 1]);
+        const V4 l = __builtin_bit_cast(V4, ((const uint2*)&phs[slot][1][kg][rbase + 16 * r + li])[lk & 1]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pr[r] += (double)(acc[r][CS][q] * ((float)h[q] + (float)l[q]));
+      }
+    }
+  };
+  issue(0);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  int st = 0;
+  auto step = [&](auto first) __attribute__((always_inline)) {
+    if (st + 1 < nst) issue(st + 1);   // into the slot every wave finished with at the last barrier
+    body(st, first);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this wave's loads of stage st + 1
+    __syncthreads();
+  };
+  auto phase = [&](auto first) __attribute__((always_inline)) {
+    for (; st < nst && first_live(st) == decltype(first)::value; ++st) step(first);
+  };
+  phase(std::integral_constant<int, 0>());
+  phase(std::integral_constant<int, 1>());
+  phase(std::integral_constant<int, 2>());
+  phase(std::integral_constant<int, 3>());
+  phase(std::integral_constant<int, 4>());
+  for (; st < nst; ++st) {   // no live tile left in this wave: it still issues its loads
+    if (st + 1 < nst) issue(st + 1);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    double p = pr[r];
+    p += __shfl_xor(p, 16);
+    p += __shfl_xor(p, 32);
+    if (lk == 0) red[hr][w][16 * r + li] = p;
+  }
+  __syncthreads();
+  if (t < kWP) {
+    const int h = t >> 7, tt = t & 127;
+    part[(int64_t)cg * ldp + row0 + t] = ((red[h][0][tt] + red[h][1][tt]) + red[h][2][tt]) + red[h][3][tt];
+  }
+}
+
+template <typename T16>
+__global__ __launch_bounds__(512) void k_contract_split_wide(const void* __restrict__ phi, int ld,
+                                                             const T16* __restrict__ p_hi,
+                                                             const T16* __restrict__ p_lo, int np, int nbf, int nrt,
+                                                             int ncg, double* __restrict__ part) {
+  int rt, cg;
+  if (!tile_coords(blockIdx.x, nrt, ncg, rt, cg)) return;
+  wide_tile<T16>(rt, cg, (const T16*)phi, ld, p_hi, p_lo, np, nbf, ncg, part, ld);
+}
+
 // Variant 3: tiles of 128 points x 64 columns with waves stacked 8 x 1 (wave w
 // owns rows 16w .. 16w+15 and all 64 columns), so every wave has the same
 // columns and, in the stages that cross the block's diagonal, all waves skip
@@ -2083,6 +2245,66 @@ void launch_split_tiles(hipStream_t s, int v, int blocks, const void* phi, int l
   }
 }
 
+// Split variant 5: basis values written pre-split, contracted by k_contract_split_wide (workgroups
+// of 256 points x 320 columns); chunks of points padded to a multiple of 256.
+template <typename T16, typename TO>
+int density_wide(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int nsub, TO* out) {
+  const int np = mol->nbf_pad;
+  const int nsub3 = nsub * nsub * nsub;
+  const int64_t nvox_tot = npoints(g);
+  int64_t maxpts = (int64_t)(kPhiBudget / ((size_t)np * sizeof(float)));
+  maxpts = std::max<int64_t>(maxpts / kWP * kWP, kWP);
+  int64_t nvox = std::max<int64_t>(maxpts / nsub3, 1);
+  nvox = std::min<int64_t>(nvox, nvox_tot);
+  nvox = std::min<int64_t>(nvox, ((int64_t)1 << 30) / nsub3);
+  const int64_t ldmax = round_up(nvox * nsub3, kWP);
+  const int ncg = wide_groups(mol->nbf);
+  void* phi = nullptr;
+  int rc = qm_ws(ctx, s, WS_PHI, sizeof(float) * (size_t)np * ldmax, &phi);
+  if (rc) return rc;
+  double* rho = nullptr;
+  if (nsub3 > 1 && (rc = qm_ws(ctx, s, WS_RHO, sizeof(double) * ldmax, (void**)&rho))) return rc;
+  double* part = nullptr;
+  if ((rc = qm_ws(ctx, s, WS_PART, sizeof(double) * (size_t)ncg * ldmax, (void**)&part))) return rc;
+  const T16* Phi = (const T16*)(std::is_same<T16, _Float16>::value ? mol->P16 : mol->Pb16);
+  const T16* Plo = Phi + (int64_t)np * np;
+  const float S = split_phi_scale(mol);
+  const GeomView gv = geom_view(g);
+  const MolView mv = mol->view();
+  const double sc = g->step * g->step * g->step / mol->pfac / ((double)S * S);
+  for (int64_t v0 = 0; v0 < nvox_tot; v0 += nvox) {
+    const int nv = (int)std::min<int64_t>(nvox, nvox_tot - v0);
+    const int npts = nv * nsub3;
+    const int ld = (int)round_up(npts, kWP);
+    {
+      ProfScope ps(ctx, s, QM_K_PHI_SPLIT);
+      hipLaunchKernelGGL(k_phi_split<T16>, dim3((ld + kPhiThreads - 1) / kPhiThreads), dim3(kPhiThreads), 0, s, mv,
+                         (const float2*)mol->prims_l2e, gv, v0, nv, nsub, npts, ld, np, S, (T16*)phi,
+                         (T16*)phi + (int64_t)np * ld);
+    }
+    QM_CHECK_LAUNCH(ctx);
+    const int nrt = ld / kWP;
+    {
+      ProfScope pc(ctx, s, QM_K_DENSITY_SPLIT);
+      hipLaunchKernelGGL(k_contract_split_wide<T16>, dim3(tile_grid(nrt, ncg)), dim3(512), 0, s, phi, ld, Phi, Plo,
+                         np, mol->nbf, nrt, ncg, part);
+    }
+    QM_CHECK_LAUNCH(ctx);
+    if (nsub3 == 1) {
+      hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncg,
+                         npts, sc, (const float*)nullptr, out, v0);
+    } else {
+      hipLaunchKernelGGL(k_tiles_finish<double>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld,
+                         ncg, npts, sc / nsub3, (const float*)nullptr, rho, (int64_t)0);
+      QM_CHECK_LAUNCH(ctx);
+      hipLaunchKernelGGL(k_subgrid_sum<TO>, dim3((nv + 255) / 256), dim3(256), 0, s, (const double*)rho, nv, nsub3,
+                         out, v0);
+    }
+    QM_CHECK_LAUNCH(ctx);
+  }
+  return QM_OK;
+}
+
 // Split variant 4: basis values evaluated inside the contraction (k_density_fused), no basis-value
 // workspace; launches of up to kFChunk points, partial sums per column group.
 template <typename T16, typename TO>
@@ -2138,6 +2360,7 @@ int density_split(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int
   if (ctx->split_tm == 4 && fused_fits(mol)) return density_fused<T16, TO>(ctx, mol, s, g, nsub, out);
   // tables too large for the fused kernel's LDS: basis values written pre-split, LDS DMA tiles
   const int variant = ctx->split_tm == 4 ? 3 : ctx->split_tm;
+  if (variant == 5) return density_wide<T16, TO>(ctx, mol, s, g, nsub, out);
   const int np = mol->nbf_pad;
   const int nsub3 = nsub * nsub * nsub;
   const int64_t nvox_tot = npoints(g);
@@ -2293,8 +2516,9 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
   int rc = qm_enter(ctx);
   if (rc) return rc;
   if (knob == QM_TUNE_SPLIT_VARIANT) {
-    QM_REQUIRE(ctx, split_tiled(value) || value == 4 || value == 64 || value == 128, QM_EINVAL,
-               "tune: split variant must be 0..3 (tiles), 4 (fused) or 64 / 128 (rows per workgroup)");
+    QM_REQUIRE(ctx, split_tiled(value) || value == 4 || value == 5 || value == 64 || value == 128, QM_EINVAL,
+               "tune: split variant must be 0..3 (tiles), 4 (fused), 5 (wide tiles) or 64 / 128 (rows per "
+               "workgroup)");
     ctx->split_tm = value;
     return QM_OK;
   }
@@ -2343,7 +2567,7 @@ int qm_autotune_density(qm_ctx* ctx, qm_mol* mol, void* stream, const qm_geom* g
   void* out = nullptr;
   if ((rc = qm_ws(ctx, s, WS_MISC, sizeof(double) * (size_t)npoints(&slab), &out))) return rc;
   const bool split = precision == QM_PREC_F16X3 || precision == QM_PREC_BF16X3;
-  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 4, 64, 128} : std::vector<int>{0, 1, 2, 3, 4};
+  std::vector<int> cands = split ? std::vector<int>{0, 1, 2, 3, 4, 5, 64, 128} : std::vector<int>{0, 1, 2, 3, 4};
   int& knob = split ? ctx->split_tm : ctx->f64_variant;
   hipEvent_t a, b;
   QM_CHECK_HIP(ctx, hipEventCreate(&a));

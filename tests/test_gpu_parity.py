@@ -122,7 +122,7 @@ def test_density_block_alignment(qm, obasis, natoms, seed):
 
 @pytest.mark.parametrize("prec,tol", [("f16", 1e-6), ("bf16", 2e-5)])
 @pytest.mark.parametrize("subgrid", [1, 2])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_density_mixed_tiled(qm, small, prec, tol, subgrid, variant):
     """The tiled split contractions (partial sums per column block; 4 evaluates the basis values inside
     the contraction kernel) against the oracle."""
@@ -432,7 +432,7 @@ def test_density_fp64_large_basis(qm, large, variant):
     assert q.raw_total == pytest.approx(raw, rel=1e-10)
 
 
-@pytest.mark.parametrize("rows", [0, 1, 2, 3, 4, 64, 128])
+@pytest.mark.parametrize("rows", [0, 1, 2, 3, 4, 5, 64, 128])
 def test_density_f16_large_basis_tiles(qm, large, rows):
     from oracle import density as od
     from qmtools import autotune
@@ -460,7 +460,7 @@ def test_autotune_cache(qm, small, tmp_path):
     _say("autotune", grid.shape)
     old = autotune.get("split_variant")
     e = autotune.autotune_density(mol, grid, "mixed", path=path)
-    assert e["split_variant"] in (0, 1, 2, 3, 4, 64, 128) and e["ms"] > 0
+    assert e["split_variant"] in (0, 1, 2, 3, 4, 5, 64, 128) and e["ms"] > 0
     table = json.load(open(path))
     assert autotune.key_for(mol, "mixed") in table
     autotune.set("split_variant", 64)

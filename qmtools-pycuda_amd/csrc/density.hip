@@ -1135,14 +1135,20 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
         bh[c] = __builtin_bit_cast(V8, ring[slot][2][lk][wc * TCW + c * 16 + li]);
         bl[c] = __builtin_bit_cast(V8, ring[slot][3][lk][wc * TCW + c * 16 + li]);
       }
+      // transposed tiles (M first); each product runs over all tiles before the next, so consecutive
+      // MFMAs are independent (a tile's three products keep their order)
 #pragma unroll
       for (int r = 0; r < RT; ++r)
 #pragma unroll
-        for (int c = cs; c < CT; ++c) {   // transposed tiles: M first
-          acc[r][c] = Vec8<T16>::mma(bh[c], al[r], acc[r][c]);
-          acc[r][c] = Vec8<T16>::mma(bl[c], ah[r], acc[r][c]);
-          acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
-        }
+        for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bh[c], al[r], acc[r][c]);
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bl[c], ah[r], acc[r][c]);
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
     };
     const int cd = 2 * (st - c0 / 32) - 4 * wc;   // first of the two tiles whose last stage this is
     const int cs = max(min(max(cd, 0), 4), cpad);
@@ -1504,12 +1510,14 @@ __device__ __forceinline__ void fused_tile(int rt, int cg, const unsigned char* 
           ahn = __builtin_bit_cast(V8, phs[slot][0][lk][16 * (r + 1) + li]);
           aln = __builtin_bit_cast(V8, phs[slot][1][lk][16 * (r + 1) + li]);
         }
+        // transposed tiles (M first); each product runs over the tiles before the next, so
+        // consecutive MFMAs are independent (a tile's three products keep their order)
 #pragma unroll
-        for (int c = CS; c < 5; ++c) {   // transposed tiles: M first
-          acc[r][c] = Vec8<T16>::mma(mh[c], al, acc[r][c]);
-          acc[r][c] = Vec8<T16>::mma(ml[c], ah, acc[r][c]);
-          acc[r][c] = Vec8<T16>::mma(mh[c], ah, acc[r][c]);
-        }
+        for (int c = CS; c < 5; ++c) acc[r][c] = Vec8<T16>::mma(mh[c], al, acc[r][c]);
+#pragma unroll
+        for (int c = CS; c < 5; ++c) acc[r][c] = Vec8<T16>::mma(ml[c], ah, acc[r][c]);
+#pragma unroll
+        for (int c = CS; c < 5; ++c) acc[r][c] = Vec8<T16>::mma(mh[c], ah, acc[r][c]);
         __builtin_amdgcn_sched_barrier(0);
         ah = ahn;
         al = aln;
@@ -1661,12 +1669,14 @@ __device__ __forceinline__ void wide_tile(int rt, int cg, const T16* __restrict_
         ahn = __builtin_bit_cast(V8, phs[slot][0][lk][rbase + 16 * (r + 1) + li]);
         aln = __builtin_bit_cast(V8, phs[slot][1][lk][rbase + 16 * (r + 1) + li]);
       }
+      // transposed tiles (M first); each product runs over the tiles before the next, so consecutive
+      // MFMAs are independent (a tile's three products still accumulate in the same order)
 #pragma unroll
-      for (int c = CS; c < 5; ++c) {   // transposed tiles: M first
-        acc[r][c] = Vec8<T16>::mma(mh[c], al, acc[r][c]);
-        acc[r][c] = Vec8<T16>::mma(ml[c], ah, acc[r][c]);
-        acc[r][c] = Vec8<T16>::mma(mh[c], ah, acc[r][c]);
-      }
+      for (int c = CS; c < 5; ++c) acc[r][c] = Vec8<T16>::mma(mh[c], al, acc[r][c]);
+#pragma unroll
+      for (int c = CS; c < 5; ++c) acc[r][c] = Vec8<T16>::mma(ml[c], ah, acc[r][c]);
+#pragma unroll
+      for (int c = CS; c < 5; ++c) acc[r][c] = Vec8<T16>::mma(mh[c], ah, acc[r][c]);
       __builtin_amdgcn_sched_barrier(0);
       ah = ahn;
       al = aln;

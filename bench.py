@@ -35,7 +35,7 @@ import numpy  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per contraction launch measured with rocprofv3 FETCH_SIZE (x2 calibration) and
-# WRITE_SIZE (x1) for this workload; source: profiles/config2_pmc_traffic.md
+# WRITE_SIZE (x1) for this workload; source: profiles/mol60_grid256_pmc_traffic.md
 # (precision, nbf_pad, variant): (points per launch, HBM bytes per launch)
 MEASURED_TRAFFIC = {("fp64", 640, "tiles_dma"): (838784, 3787436 * 1024 * 2 + 32765 * 1024),
                     ("fp64", 640, "tiles"): (838784, 6138183 * 1024 * 2 + 32765 * 1024),
@@ -583,7 +583,7 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "density contraction (Phi P Phi^T diagonal)",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic,
-                     "traffic_source": "profiles/config2_pmc_traffic.md",
+                     "traffic_source": "profiles/mol60_grid256_pmc_traffic.md",
                      "algorithmic": (f"({'2 nbf^2' if dense else 'nbf (nbf + 1)'} + 2 nbf) flops/point x "
                                      f"{launch_pts:.0f} points/launch"
                                      + (" x 3 split products" if mult > 1 else "")

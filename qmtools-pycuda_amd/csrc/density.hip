@@ -645,6 +645,12 @@ __device__ __forceinline__ void lds_dma16(const void* gsrc, const void* lds_wave
   asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(l), "v"(gsrc) : "memory");
 }
 
+// saddr form of lds_dma16: wave-uniform 64-bit base (SGPRs) plus a 32-bit byte offset per lane;
+// lds_addr is the wave's LDS byte address (M0).
+__device__ __forceinline__ void lds_dma16_s(const void* sbase, unsigned voff, unsigned lds_addr) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(lds_addr), "v"(voff), "s"(sbase) : "memory");
+}
+
 // k per stage of the DMA-fed FP64 tiles (ring of 32 / kF64Ks stages).  Stages of 8 k in a ring
 // of 4 (one more stage of lookahead, twice the barriers) measured 4% slower than 16 k in a ring of 2.
 constexpr int kF64Ks = 16;
@@ -1068,9 +1074,21 @@ __global__ __launch_bounds__(64 * WR * WC) void k_contract_split_tiles(const voi
 // finished once it passes stage st's barrier.  The loads are issued from inline
 // assembly (M0 holds the LDS address) so that the compiler does not wait for
 // them before each LDS read; the explicit s_waitcnt vmcnt before the barrier
-// waits for them instead.  No other code in this kernel uses M0.  Tiles and the
-// transposed MFMA order are those of tiled_split_body<PRE>; its row-dot sums are
+// waits for them instead.  No other code in this kernel uses M0.  The transposed
+// MFMA order per tile is that of tiled_split_body<PRE>; the row-dot sums are
 // formed here in the K loop, from the LDS copy of the basis values (below).
+//
+// The stage loop is split so that its code has no runtime tile selection: the
+// stages below the block's diagonal run every tile with no row-dot, and the four
+// diagonal stages are unrolled with compile-time first tile and row-dot tile.
+// The two waves of a SIMD take interleaved 16-column tiles, so both drop out of
+// the triangle alike and no diagonal stage branches on the wave.  (A version that
+// selected tiles at run time spent 43 VALU instructions per 24 MFMAs in a full
+// stage, 16 of them register moves; this one spends 4, with 80 VGPRs.)  Measured
+// and dropped: persistent workgroups that prefetch the next tile's first stage
+// (slower: the row tiles' column blocks drift apart); a three- or four-slot ring
+// at one workgroup per CU (two waves per SIMD: 35% slower); warming L2 with a
+// dummy DMA of stage st + 2 (10% slower).
 
 template <typename T16>
 __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __restrict__ a_hi, int ld,
@@ -1082,105 +1100,125 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
   // ring slot: 4 operand planes (A hi, A lo, B hi, B lo) of [4 k groups][128] 16-byte words
   __shared__ u32x4 ring[2][4][4][kTT];
   __shared__ double red[WC][kTT];
-  const T16* __restrict__ a_lo = a_hi + (int64_t)np * ld;
+  const char* __restrict__ a_hi_b = (const char*)a_hi;
+  const char* __restrict__ a_lo_b = (const char*)(a_hi + (int64_t)np * ld);
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int wr = wave & 3, wc = wave >> 2;   // waves w, w + 4 (one SIMD) take the two column halves
+  const int wr = wave & 3, wc = wave >> 2;   // waves w, w + 4 share a SIMD
   const int li = lane & 15, lk = lane >> 4;
   // column blocks right-aligned on ne = nbf rounded up to 32 (as in tiled_f64_dma_body): only
   // block 0 holds padding, as columns below 0, skipped like tiles below the diagonal
   const int ne = (nbf + 31) / 32 * 32;
   const int row0 = rt * kTT, c0 = cb * kTT - (ncb * kTT - ne);
   const int nst = (c0 + kTT) / 32;
-  const int cpad = min(max(-(c0 + 64 * wc) / 16, 0), 4);   // tile columns of this wave below column 0
-  // thread t fetches word t of each plane: k group g = t / 128, row / column ii = t % 128
+  const int jd = c0 / 32;   // c0 is a multiple of 32: stage jd + j is the block's diagonal stage j
+  // thread t fetches word t of each plane: k group g = t / 128, row / column ii = t % 128.  The
+  // stage's base address is wave-uniform (SGPRs, saddr form); the lane's byte offset within the
+  // stage is at most 64 ld < 2^32 (a plane of basis values is at most 2 GiB).
   const int g = t / kTT, ii = t % kTT;
   const int bcol = min(max(c0 + ii, 0), np - 1);   // clamped columns are never used
+  const unsigned oa = (unsigned)(g * ld + ii) * 16u;
+  const unsigned ob = (unsigned)(g * np + bcol) * 16u;
+  const unsigned lring = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)&ring[0][0][0][0] + 1024u * wave;
   auto issue = [&](int st) __attribute__((always_inline)) {
-    const int slot = st & 1;
-    const int64_t oa = ((int64_t)(st * 4 + g) * ld + row0 + ii) * 8;
-    const int64_t ob = ((int64_t)(st * 4 + g) * np + bcol) * 8;
-    lds_dma16(a_hi + oa, &ring[slot][0][0][0] + 64 * wave);
-    lds_dma16(a_lo + oa, &ring[slot][1][0][0] + 64 * wave);
-    lds_dma16(p_hi + ob, &ring[slot][2][0][0] + 64 * wave);
-    lds_dma16(p_lo + ob, &ring[slot][3][0][0] + 64 * wave);
+    const unsigned l = lring + (unsigned)(st & 1) * (unsigned)sizeof(ring[0]);
+    // 4 k groups of 16 bytes per stage; the tile's first row goes into the uniform base
+    const int64_t sa = (int64_t)st * 64 * ld + (int64_t)row0 * 16, sb = (int64_t)st * 64 * np;
+    lds_dma16_s(a_hi_b + sa, oa, l);
+    lds_dma16_s(a_lo_b + sa, oa, l + (unsigned)sizeof(ring[0][0]));
+    lds_dma16_s((const char*)p_hi + sb, ob, l + 2u * (unsigned)sizeof(ring[0][0]));
+    lds_dma16_s((const char*)p_lo + sb, ob, l + 3u * (unsigned)sizeof(ring[0][0]));
   };
   f4 acc[RT][CT];
   double pr[RT];   // this lane's row-dot partial sums (rows wr TRW + 16 r + li)
+  // Wave (wr, wc) owns rows wr TRW .. + TRW - 1 and the interleaved 16-column tiles 2 c + wc, c =
+  // 0 .. CT - 1, of the block (columns c0 + 32 c + 16 wc ..), so that both waves of a SIMD drop
+  // tiles out of the triangle alike.  MFMAs of one stage for the tiles first .. CT - 1:
+  auto body = [&](int slot, auto first) __attribute__((always_inline)) {
+    constexpr int cs = decltype(first)::value;
+    V8 ah[RT], al[RT], bh[CT], bl[CT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+      ah[r] = __builtin_bit_cast(V8, ring[slot][0][lk][wr * TRW + r * 16 + li]);
+      al[r] = __builtin_bit_cast(V8, ring[slot][1][lk][wr * TRW + r * 16 + li]);
+    }
+#pragma unroll
+    for (int c = cs; c < CT; ++c) {
+      bh[c] = __builtin_bit_cast(V8, ring[slot][2][lk][32 * c + 16 * wc + li]);
+      bl[c] = __builtin_bit_cast(V8, ring[slot][3][lk][32 * c + 16 * wc + li]);
+    }
+    // transposed tiles (M first); each product runs over all tiles before the next, so consecutive
+    // MFMAs are independent (a tile's three products keep their order)
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+      for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bh[c], al[r], acc[r][c]);
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+      for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bl[c], ah[r], acc[r][c]);
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+      for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
+  };
+  // Row-dot of the tile a stage completes: the wave's tile c (columns n0 = c0 + 32 c + 16 wc ..)
+  // gets its last products in stage (n0 + 15) / 32 = diagonal stage c, whose A operand in the ring
+  // slot holds the basis values of exactly its columns (k = n).  Lane (li, lk), register q of a
+  // tile is X[t = li][n = 4 lk + q]: the 4 values of one point sit in one 8-byte word of each
+  // plane.  Each wave sums its tiles in column order.
+  auto rowdot = [&](int slot, auto tile) __attribute__((always_inline)) {
+    constexpr int c = decltype(tile)::value;
+    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+    typedef typename std::conditional<std::is_same<T16, _Float16>::value, h4, b4>::type V4;
+    const int kg = 2 * wc + (lk >> 1);   // k group of n - 32 st = 16 wc + 4 lk + q
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+      const int row = wr * TRW + r * 16 + li;
+      const V4 h = __builtin_bit_cast(V4, ((const uint2*)&ring[slot][0][kg][row])[lk & 1]);
+      const V4 l = __builtin_bit_cast(V4, ((const uint2*)&ring[slot][1][kg][row])[lk & 1]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) pr[r] += (double)(acc[r][c][q] * ((float)h[q] + (float)l[q]));
+    }
+  };
 #pragma unroll
   for (int r = 0; r < RT; ++r) {
     pr[r] = 0.0;
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[r][c] = f4{0, 0, 0, 0};
   }
-  issue(0);
-  for (int st = 0; st < nst; ++st) {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this thread's loads of stage st
-    __syncthreads();                                    // everyone's loads of st; everyone done with st - 1
+  // opens stage st: this thread's loads of st have landed, every thread's have after the barrier,
+  // and every wave is done with st - 1, whose slot receives st + 1
+  auto open = [&](int st) __attribute__((always_inline)) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
     if (st + 1 < nst) issue(st + 1);
-    const int slot = st & 1;
-    // M is zero below its diagonal: in the block's diagonal stages (j = st - c0 / 32 >= 0) tile
-    // column c (columns c0 + 64 wc + 16 c ..) needs this stage only while 32 j < 64 wc + 16 c + 16,
-    // i.e. c >= 2 j - 4 wc; a tile computed anyway only adds products with M's zeros.
-    auto body = [&](auto first) __attribute__((always_inline)) {
-      constexpr int cs = decltype(first)::value;
-      V8 ah[RT], al[RT], bh[CT], bl[CT];
-#pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        ah[r] = __builtin_bit_cast(V8, ring[slot][0][lk][wr * TRW + r * 16 + li]);
-        al[r] = __builtin_bit_cast(V8, ring[slot][1][lk][wr * TRW + r * 16 + li]);
-      }
-#pragma unroll
-      for (int c = cs; c < CT; ++c) {
-        bh[c] = __builtin_bit_cast(V8, ring[slot][2][lk][wc * TCW + c * 16 + li]);
-        bl[c] = __builtin_bit_cast(V8, ring[slot][3][lk][wc * TCW + c * 16 + li]);
-      }
-      // transposed tiles (M first); each product runs over all tiles before the next, so consecutive
-      // MFMAs are independent (a tile's three products keep their order)
-#pragma unroll
-      for (int r = 0; r < RT; ++r)
-#pragma unroll
-        for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bh[c], al[r], acc[r][c]);
-#pragma unroll
-      for (int r = 0; r < RT; ++r)
-#pragma unroll
-        for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bl[c], ah[r], acc[r][c]);
-#pragma unroll
-      for (int r = 0; r < RT; ++r)
-#pragma unroll
-        for (int c = cs; c < CT; ++c) acc[r][c] = Vec8<T16>::mma(bh[c], ah[r], acc[r][c]);
-    };
-    const int cd = 2 * (st - c0 / 32) - 4 * wc;   // first of the two tiles whose last stage this is
-    const int cs = max(min(max(cd, 0), 4), cpad);
-    if (cs < 2) body(std::integral_constant<int, 0>());
-    else if (cs < 4) body(std::integral_constant<int, 2>());
-    // Row-dot of the tiles this stage completes: tile c (columns n0 = c0 + 64 wc + 16 c ..) gets its
-    // last products in stage (n0 + 15) / 32, whose A operand in the ring slot holds the basis
-    // values of exactly its columns (k = n).  Lane (li, lk), register q of a tile is X[t = li][n =
-    // 4 lk + q]: the 4 values of one point sit in one 8-byte word of each plane.  Tiles are summed
-    // in column order, as the epilogue of tiled_split_body<PRE> does from global memory.
-    auto rowdot = [&](auto first) __attribute__((always_inline)) {
-      constexpr int c1 = decltype(first)::value;
-      typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-      typedef __bf16 b4 __attribute__((ext_vector_type(4)));
-      typedef typename std::conditional<std::is_same<T16, _Float16>::value, h4, b4>::type V4;
-#pragma unroll
-      for (int dc = 0; dc < 2; ++dc) {
-        const int kg = 2 * dc + (lk >> 1);   // k group of n - 32 st = 16 dc + 4 lk + q
-#pragma unroll
-        for (int r = 0; r < RT; ++r) {
-          const int row = wr * TRW + r * 16 + li;
-          const V4 h = __builtin_bit_cast(V4, ((const uint2*)&ring[slot][0][kg][row])[lk & 1]);
-          const V4 l = __builtin_bit_cast(V4, ((const uint2*)&ring[slot][1][kg][row])[lk & 1]);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) pr[r] += (double)(acc[r][c1 + dc][q] * ((float)h[q] + (float)l[q]));
-        }
-      }
-    };
-    // cpad is 0, 2 or 4 (c0 is a multiple of 32): a pair of tiles is all padding or none
-    if (cd == 0 && cpad == 0) rowdot(std::integral_constant<int, 0>());
-    else if (cd == 2 && cpad <= 2) rowdot(std::integral_constant<int, 2>());
+  };
+  issue(0);
+  // stages below the block's diagonal (none in block 0): every tile, no row-dot
+  for (int st = 0; st < jd; ++st) {
+    open(st);
+    body(st & 1, std::integral_constant<int, 0>());
   }
+  // The four diagonal stages, with compile-time tiles.  M is zero below its diagonal: in diagonal
+  // stage j, the wave's tile c (columns c0 + 32 c + 16 wc ..) still needs products while 32 j <
+  // 32 c + 16 wc + 16, i.e. c >= j, and tile j gets its last ones.  Block 0 starts at stage 0 =
+  // diagonal stage -jd; its tiles below column 0 are those below -jd, skipped too.
+  auto diag = [&](auto jc) __attribute__((always_inline)) {
+    constexpr int J = decltype(jc)::value;
+    const int st = jd + J;
+    if (st < 0) return;
+    open(st);
+    body(st & 1, jc);
+    __builtin_amdgcn_sched_barrier(0);
+    rowdot(st & 1, jc);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  diag(std::integral_constant<int, 0>());
+  diag(std::integral_constant<int, 1>());
+  diag(std::integral_constant<int, 2>());
+  diag(std::integral_constant<int, 3>());
 #pragma unroll
   for (int r = 0; r < RT; ++r) {
     double p = pr[r];
@@ -1193,7 +1231,7 @@ __device__ __forceinline__ void tiled_dma_body(int rt, int cb, const T16* __rest
 }
 
 template <typename T16>
-__global__ __launch_bounds__(512) void k_contract_split_dma(const void* __restrict__ phi, int ld,
+__global__ __launch_bounds__(512, 4) void k_contract_split_dma(const void* __restrict__ phi, int ld,
                                                             const T16* __restrict__ p_hi, const T16* __restrict__ p_lo,
                                                             int np, int nbf, int nrt, int ncb,
                                                             double* __restrict__ part) {
@@ -1593,7 +1631,8 @@ __global__ __launch_bounds__(512) void k_density_fused(MolView m, const float2* 
 // issued right after the barrier that opens stage st.  MFMA order, products
 // and row-dot are those of the fused kernel's consumers.  LDS: 144 KB + 8 KB.
 //
-// Measured on the 60-atom, 256^3 workload: bitwise the results of variant 3, but 30.4 ms of
+// Measured on the 60-atom, 256^3 workload: the results of variant 3 to 2.5e-16 relative (the
+// row-dot terms are grouped differently), but 30.4 ms of
 // contraction against 17.7 ms.  Counters (rocprofv3, same MFMA busy cycles in both): it waits
 // on LDS instruction issue 4.5x less than variant 3, but with one workgroup (two waves) per
 // SIMD pair the barrier and the fragment loads at the start of each stage leave the MFMA pipes
@@ -2775,7 +2814,7 @@ __global__ __launch_bounds__(64 * WR * WC) void kb_contract_split_tiles(const BC
 }
 
 template <typename T16>
-__global__ __launch_bounds__(512) void kb_contract_split_dma(const BChunk* __restrict__ c, int n) {
+__global__ __launch_bounds__(512, 4) void kb_contract_split_dma(const BChunk* __restrict__ c, int n) {
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
   int rt, cb;
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;

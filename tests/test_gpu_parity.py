@@ -173,13 +173,18 @@ def test_seed(qm, small):
     assert math.fsum(s.qube.ravel()) == pytest.approx(float(mol.qtot), rel=1e-13)
 
 
-def test_vne_classic(qm, small):
+@pytest.mark.parametrize("divisions,n", [(2, 32), (4, 32), (16, 16)])
+def test_vne_classic(qm, small, divisions, n):
+    """divisions 4 is the default of Compute_VNe_classic (reference qmtools.py:2750), 16 what the
+    reference's scripts/test-vneclassic.py:24 uses."""
     from oracle import vne as ov
-    mol, grid, _ = small
+    from qmtools import synth
+    mol = small[0]
+    grid = synth.centred_grid(mol, n, step_bohr=0.35, pad_ang=2.0)
     nx, ny, nz, h, o = grid.geometry()
-    _say("vne classic", grid.shape)
-    v = qm.QMTools.Compute_VNe_classic(grid, mol, divisions=2)
-    want = ov.vne_classic(mol.types, mol.coords, nx, ny, nz, h, o, 2)
+    _say("vne classic", grid.shape, "divisions", divisions)
+    v = qm.QMTools.Compute_VNe_classic(grid, mol, divisions=divisions)
+    want = ov.vne_classic(mol.types, mol.coords, nx, ny, nz, h, o, divisions)
     assert _rel(v.qube, want) <= 1e-12
 
 

@@ -8,7 +8,9 @@ import math
 
 import numpy
 import pytest
-from conftest import BASIS
+import os
+
+from conftest import BASIS, MOL29766
 
 from oracle import density as od
 from oracle import reduce as orr
@@ -111,6 +113,65 @@ def test_density_integrates_to_trace_PS(oracle_basis):
     q = od.density_grid_raw(sysm, n, n, n, h, origin)
     want = 2 * numpy.sum(lower_symmetric(sysm.dm) * S)
     assert abs(q.sum() - want) / abs(want) < 2e-6
+
+
+def _overlap_gauss_hermite(system, harmonic=od.solid_harmonic):
+    """S_pq exactly, with no Cartesian expansion: a product of two primitives is
+    K exp(-p |r - P|^2) times S_La(r - A) S_Lb(r - B), a polynomial of degree <= La + Lb
+    <= 4 per axis, which 3-point Gauss-Hermite per axis integrates exactly; the
+    polynomial is evaluated through `harmonic` at the 27 nodes."""
+    xg, wg = numpy.polynomial.hermite.hermgauss(3)
+    X, Y, Z = numpy.meshgrid(xg, xg, xg, indexing="ij")
+    W = (wg[:, None, None] * wg[None, :, None] * wg[None, None, :]).ravel()
+    nodes = numpy.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    offs = numpy.cumsum([0] + [2 * L + 1 for (_, L, _, _) in system.shells])
+    S = numpy.zeros((system.nbf, system.nbf))
+    for sa, (ia, La, ala, coa) in enumerate(system.shells):
+        A = system.coords[ia]
+        for sb, (ib, Lb, alb, cob) in enumerate(system.shells):
+            B = system.coords[ib]
+            blk = numpy.zeros((2 * La + 1, 2 * Lb + 1))
+            for a, ca in zip(ala, coa):
+                for b, cb in zip(alb, cob):
+                    p = a + b
+                    r = (a * A + b * B) / p + nodes / math.sqrt(p)
+                    K = math.exp(-a * b / p * numpy.sum((A - B) ** 2))
+                    ra, rb = r - A, r - B
+                    fa = numpy.array([harmonic(La, m, ra[:, 0], ra[:, 1], ra[:, 2]) for m in od.M_ORDER[La]])
+                    fb = numpy.array([harmonic(Lb, m, rb[:, 0], rb[:, 1], rb[:, 2]) for m in od.M_ORDER[Lb]])
+                    blk += ca * cb * K * p ** -1.5 * (fa * W) @ fb.T
+            S[offs[sa]:offs[sa + 1], offs[sb]:offs[sb + 1]] = blk
+    return S
+
+
+def test_ccsd_density_matrix_trace_pins_basis_convention(oracle_basis):
+    """The reference's own molecule (molecule_29766_0: Br, 2 Cl, 6 C, 3 H; 162 cc-pVDZ functions
+    with s, p and d shells) ships a CCSD density matrix computed in the basis convention the
+    reference's kernels use.  Its electron count 2 Tr(P S) must be sum Z = 108 exactly when the
+    oracle's basis functions (order of m, solid-harmonic forms and signs, contraction
+    normalisation) are those of the matrix.  Swapping the d functions m = +-1, or flipping the
+    sign of d(x^2 - y^2), moves the count by 0.3 to 0.7 electrons."""
+    xyz = numpy.loadtxt(os.path.join(MOL29766, "GEOM-B3LYP.xyz"))
+    dm = numpy.load(os.path.join(MOL29766, "D-CCSD.npy"), allow_pickle=False)
+    ang = 1.8897261246257702
+    sysm = make_system(xyz[:, 0].astype(int), xyz[:, 1:] * ang, dm, oracle_basis)
+    P = lower_symmetric(dm)
+    nz = float(xyz[:, 0].sum())
+
+    def count(harmonic):
+        return 2.0 * numpy.sum(P * _overlap_gauss_hermite(sysm, harmonic))
+
+    assert abs(count(od.solid_harmonic) - nz) < 1e-8 * nz
+
+    def swapped(L, m, x, y, z):
+        return od.solid_harmonic(L, -m if (L == 2 and abs(m) == 1) else m, x, y, z)
+
+    def flipped(L, m, x, y, z):
+        v = od.solid_harmonic(L, m, x, y, z)
+        return -v if (L == 2 and m == 2) else v
+
+    assert abs(count(swapped) - nz) > 0.1
+    assert abs(count(flipped) - nz) > 0.1
 
 
 def test_single_s_primitive_closed_form():

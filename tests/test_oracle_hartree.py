@@ -10,8 +10,8 @@ from oracle import fft as off
 from oracle import gto as og
 from oracle import jacobi as oj
 from oracle import vne as ov
-from oracle.density import C00
-from oracle.system import make_system
+from oracle.density import C00, M_ORDER, solid_harmonic
+from oracle.system import lower_symmetric, make_system
 
 # ----------------------------------------------------------------------------- Jacobi
 
@@ -264,6 +264,56 @@ def test_gto_mcmurchie_davidson_vs_quadrature():
     pts = numpy.array([[0.35, -0.2, 0.25], [1.5, 0.4, -0.8], [-0.7, 0.9, 0.3]])
     v = og.potential(sysm, pts, nuclei=False)
     w = og.potential_quadrature(sysm, pts, nt=160, ngh=10, nuclei=False)
+    numpy.testing.assert_allclose(v, w, rtol=1e-9)
+
+
+def _potential_solid_harmonic_quadrature(system, pts, nt=160):
+    """V_H at pts by a route that shares nothing with the Hermite algebra or its Cartesian
+    expansion: 1/|r - C| = (2/sqrt(pi)) int exp(-t^2 |r - C|^2) dt (t = s/(1-s), Gauss-Legendre
+    in s); for fixed t the integrand is one Gaussian exp(-g |r - P_t|^2) times S_La(r - A)
+    S_Lb(r - B), of degree <= 4 per axis, integrated exactly on a 3 x 3 x 3 Gauss-Hermite grid
+    with the solid harmonics evaluated at the nodes."""
+    xg, wg = numpy.polynomial.hermite.hermgauss(3)
+    X, Y, Z = numpy.meshgrid(xg, xg, xg, indexing="ij")
+    W = (wg[:, None, None] * wg[None, :, None] * wg[None, None, :]).ravel()
+    nodes = numpy.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    sl, wl = numpy.polynomial.legendre.leggauss(nt)
+    s = 0.5 * (sl + 1.0)
+    tq = s / (1.0 - s)
+    wt = 0.5 * wl / (1.0 - s) ** 2
+    Pt = 2.0 * lower_symmetric(system.dm)
+    offs = numpy.cumsum([0] + [2 * L + 1 for (_, L, _, _) in system.shells])
+    out = numpy.zeros(len(pts))
+    for n, C in enumerate(numpy.asarray(pts, dtype=numpy.float64)):
+        acc = 0.0
+        for sa, (ia, La, ala, coa) in enumerate(system.shells):
+            A = system.coords[ia]
+            for sb, (ib, Lb, alb, cob) in enumerate(system.shells):
+                B = system.coords[ib]
+                blk = Pt[offs[sa]:offs[sa + 1], offs[sb]:offs[sb + 1]]
+                for a, ca in zip(ala, coa):
+                    for b, cb in zip(alb, cob):
+                        g = a + b + tq * tq                                          # (nt,)
+                        Pc = (a * A + b * B + (tq * tq)[:, None] * C) / g[:, None]   # (nt, 3)
+                        K = numpy.exp(-(a * b * numpy.sum((A - B) ** 2) + a * tq * tq * numpy.sum((A - C) ** 2)
+                                        + b * tq * tq * numpy.sum((B - C) ** 2)) / g)
+                        r = Pc[:, None, :] + nodes[None, :, :] / numpy.sqrt(g)[:, None, None]   # (nt, 27, 3)
+                        ra, rb = r - A, r - B
+                        fa = numpy.array([solid_harmonic(La, m, ra[..., 0], ra[..., 1], ra[..., 2]) for m in M_ORDER[La]])
+                        fb = numpy.array([solid_harmonic(Lb, m, rb[..., 0], rb[..., 1], rb[..., 2]) for m in M_ORDER[Lb]])
+                        integ = numpy.einsum("atn,btn,n->abt", fa, fb, W) * (K * wt * g ** -1.5)
+                        acc += ca * cb * numpy.sum(blk[:, :, None] * integ)
+        out[n] = -acc * 2.0 / math.sqrt(math.pi)
+    return out
+
+
+def test_gto_potential_vs_solid_harmonic_quadrature():
+    """The Hermite/Boys potential against a quadrature built on the solid harmonics themselves
+    (no shared Cartesian expansion), all (La, Lb) classes up to D-D, unsymmetric P."""
+    sysm = _small_spd_system()
+    pts = numpy.array([[0.35, -0.2, 0.25], [1.5, 0.4, -0.8], [-0.7, 0.9, 0.3]])
+    v = og.potential(sysm, pts, nuclei=False)
+    w = _potential_solid_harmonic_quadrature(sysm, pts)
     numpy.testing.assert_allclose(v, w, rtol=1e-9)
 
 

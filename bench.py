@@ -284,7 +284,7 @@ def cold_request(basis):
 
 
 def jacobi_section(w, sweeps=200):
-    """Jacobi sweeps on the config-2 grid (256^3): ms per sweep and HBM roofline fraction."""
+    """Jacobi sweeps on the 256^3 grid of the 60-atom workload: ms per sweep and HBM roofline fraction."""
     import torch
     import qmtools
     from qmtools import _native as nat
@@ -331,9 +331,10 @@ def vne_section(w, pairs=25):
 
 
 def batched_section(basis, count, sweeps, seed=41):
-    """Config 4 (reduced count): mixed batched fused launches vs the plain FP64 path (one molecule at a
-    time through the drop-in API), for each Poisson method: Jacobi (hipGraph, `sweeps` sweeps, multipole
-    boundary) and free-space FFT.  Same method and sweep count on both sides of each ratio."""
+    """A batch of molecules (5-80 atoms, 64^3-192^3 grids): mixed batched fused launches vs the plain
+    FP64 path (one molecule at a time through the drop-in API), for each Poisson method: Jacobi
+    (hipGraph, `sweeps` sweeps, multipole boundary) and free-space FFT.  Same method and sweep count
+    on both sides of each ratio."""
     import torch
     import qmtools
     from qmtools import batch, synth

@@ -378,14 +378,15 @@ def serve_pool(basis, count=250, seed=77, grid_n=(64, 96, 128, 160, 192, 224, 25
     return list(zip(*synth.molecule_batch(count, (5, 120), grid_n, (0.1, 0.15), seed, basis)))
 
 
-def served_section(pool, nreq, rate, sweeps, precision="mixed", poisson="fft", seed=5):
+def served_section(pool, nreq, rate, sweeps, precision="mixed", poisson="fft", seed=5, round_sweeps=None):
     """Poisson arrivals at `rate` req/s into the continuous-batching server; each request draws a
-    molecule and its grid from `pool`.  Results are dropped on completion (throughput and latency only)."""
+    molecule and its grid from `pool`.  Results are dropped on completion (throughput and latency only).
+    round_sweeps: Jacobi lanes admit between rounds of that many sweeps (None: whole batches)."""
     from qmtools import serve
     rng = numpy.random.default_rng(seed)
     items = [pool[int(i)] for i in rng.integers(0, len(pool), size=nreq)]
     srv = serve.Server(nlanes=2, precision=precision, max_batch=8, sweeps=sweeps, hbm_budget=160 << 30,
-                       poisson=poisson, results="drop").start()
+                       poisson=poisson, results="drop", round_sweeps=round_sweeps).start()
     for m, g in pool[:2]:                                            # warm-up
         srv.submit(m, g).wait(600)
     srv.completed.clear()
@@ -397,6 +398,8 @@ def served_section(pool, nreq, rate, sweeps, precision="mixed", poisson="fft", s
     st = srv.stats()
     st.update({"rate_req_per_s": rate, "precision": precision, "poisson": poisson,
                "sweeps": sweeps if poisson == "jacobi" else None, "wall_s": time.perf_counter() - t0})
+    if poisson == "jacobi":
+        st.update({"round_sweeps": srv.round_sweeps, "joined_running": srv.joined_running})
     return st
 
 
@@ -544,7 +547,7 @@ def main():
             log(f"batched ({k}): {batched[k]['speedup']:.2f}x plain FP64, errors {batched[k]['max_rel_err_vs_fp64']}")
     if args.serve > 0 or args.serve_jacobi > 0:
         pool = serve_pool(w.basis)
-        served = {"fft_mixed": [], "fft_plain_fp64": None, "jacobi": None,
+        served = {"fft_mixed": [], "fft_plain_fp64": None, "jacobi": None, "jacobi_rounds": None,
                   "requests": "Poisson arrivals; molecules of 5-120 atoms on 64^3-256^3 grids drawn from a "
                               "seeded pool of 250"}
         if args.serve > 0:
@@ -559,6 +562,10 @@ def main():
             served["jacobi"] = [served_section(pool, args.serve_jacobi, 10.0, args.sweeps, precision=p,
                                                poisson="jacobi") for p in ("mixed", "fp64")]
             log(f"served jacobi: {served['jacobi']}")
+            # the same arrivals with admission between rounds of 200 sweeps
+            served["jacobi_rounds"] = served_section(pool, args.serve_jacobi, 10.0, args.sweeps, precision="mixed",
+                                                     poisson="jacobi", round_sweeps=200)
+            log(f"served jacobi, rounds: {served['jacobi_rounds']}")
         del pool
 
     value = w.N / (ms_step / 1000.0)

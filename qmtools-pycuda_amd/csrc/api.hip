@@ -105,6 +105,10 @@ int qm_ctx_destroy(qm_ctx* ctx) {
   (void)hipStreamSynchronize(nullptr);
   if (ctx->host_scalars) (void)hipHostFree(ctx->host_scalars);
   qm_reclaim_graphs(ctx, true);
+  if (ctx->jgraph_exec) (void)hipGraphExecDestroy(ctx->jgraph_exec);
+  if (ctx->jgraph) (void)hipGraphDestroy(ctx->jgraph);
+  if (ctx->table_uploaded) (void)hipEventDestroy(ctx->table_uploaded);
+  if (ctx->table_pinned) (void)hipHostFree(ctx->table_pinned);
   qm_fft_cache_free(ctx->fft);
   clear_records(ctx);
   if (ctx->graph_stream) (void)hipStreamDestroy(ctx->graph_stream);

@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <cstring>
+
 #include "qm_internal.h"
 
 namespace {
@@ -890,21 +892,42 @@ __global__ __launch_bounds__(kBX* kBY) void kb_copy_back(const JItem* __restrict
 
 }  // namespace
 
-// destroy the graphs whose last replay has finished (all of them if wait)
+// destroy the graphs (and events) whose last replay has finished (all of them if wait); an entry
+// with no graph only carries the event of a replay of the cached batch graph
 void qm_reclaim_graphs(qm_ctx* ctx, bool wait) {
   auto& v = ctx->graphs;
   size_t k = 0;
   for (size_t i = 0; i < v.size(); ++i) {
     if (wait) (void)hipEventSynchronize(v[i].done);
     if (hipEventQuery(v[i].done) == hipSuccess) {
-      (void)hipGraphExecDestroy(v[i].x);
-      (void)hipGraphDestroy(v[i].g);
+      if (v[i].x) (void)hipGraphExecDestroy(v[i].x);
+      if (v[i].g) (void)hipGraphDestroy(v[i].g);
       (void)hipEventDestroy(v[i].done);
     } else {
       v[k++] = v[i];
     }
   }
   v.resize(k);
+}
+
+// Copies a host item table to dtab on stream s through the context's pinned staging buffer: the
+// copy is stream-ordered and the host only waits (if at all) for the previous upload to have left
+// the staging buffer.
+static int upload_table(qm_ctx* ctx, hipStream_t s, void* dtab, const void* table, size_t bytes) {
+  if (!ctx->table_uploaded) QM_CHECK_HIP(ctx, hipEventCreateWithFlags(&ctx->table_uploaded, hipEventDisableTiming));
+  else QM_CHECK_HIP(ctx, hipEventSynchronize(ctx->table_uploaded));
+  if (ctx->table_pinned_bytes < bytes) {
+    if (ctx->table_pinned) QM_CHECK_HIP(ctx, hipHostFree(ctx->table_pinned));
+    ctx->table_pinned = nullptr;
+    ctx->table_pinned_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 64 << 10);
+    QM_CHECK_HIP(ctx, hipHostMalloc(&ctx->table_pinned, want, hipHostMallocDefault));
+    ctx->table_pinned_bytes = want;
+  }
+  std::memcpy(ctx->table_pinned, table, bytes);
+  QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, ctx->table_pinned, bytes, hipMemcpyHostToDevice, s));
+  QM_CHECK_HIP(ctx, hipEventRecord(ctx->table_uploaded, s));
+  return QM_OK;
 }
 
 extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* geoms, const void* const* rho,
@@ -983,9 +1006,9 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
     }
   }
   void* dtab = nullptr;
-  if ((rc = qm_ws(ctx, s, WS_BATCH, sizeof(JItem) * items.size(), &dtab))) return rc;
-  QM_CHECK_HIP(ctx, hipMemcpyAsync(dtab, items.data(), sizeof(JItem) * items.size(), hipMemcpyHostToDevice, s));
-  QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // pageable source
+  const size_t tbytes = sizeof(JItem) * items.size();
+  if ((rc = qm_ws(ctx, s, WS_BATCH, tbytes, &dtab))) return rc;
+  if ((rc = upload_table(ctx, s, dtab, items.data(), tbytes))) return rc;
   const JItem* dt = (const JItem*)dtab;
   const dim3 grid(blocks), sgrid(sblocks), tgrid(tblocks), thr(kBX, kBY), sthr(kSW, kSWaves);
   // one block of work on stream t; kf = sweeps in the block
@@ -1023,29 +1046,49 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
     QM_CHECK_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     QM_CHECK_HIP(ctx, hipEventRecord(ev, s));
     QM_CHECK_HIP(ctx, hipStreamWaitEvent(gs, ev, 0));
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    QM_CHECK_HIP(ctx, hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_block(gs, block);
-    hipError_t ce = hipStreamEndCapture(gs, &graph);
-    if (rc || ce != hipSuccess) {
-      (void)hipEventDestroy(ev);
-      if (graph) (void)hipGraphDestroy(graph);
-      return qm_fail(ctx, QM_EHIP, "jacobi_batch: graph capture failed: %s", hipGetErrorString(ce));
+    // every kernel argument and grid of the block follows from the item table, its address, the
+    // block length and the precision
+    std::vector<char> key(sizeof(int) * 3 + sizeof(void*) + tbytes);
+    {
+      const int hdr[3] = {n, block, f64 ? 1 : 0};
+      std::memcpy(key.data(), hdr, sizeof(hdr));
+      std::memcpy(key.data() + sizeof(hdr), &dtab, sizeof(void*));
+      std::memcpy(key.data() + sizeof(hdr) + sizeof(void*), items.data(), tbytes);
     }
-    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      (void)hipEventDestroy(ev);
-      return qm_fail(ctx, QM_EHIP, "jacobi_batch: graph instantiation failed: %s", hipGetErrorString(ie));
+    if (!ctx->jgraph_exec || key != ctx->jgraph_key) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      QM_CHECK_HIP(ctx, hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_block(gs, block);
+      hipError_t ce = hipStreamEndCapture(gs, &graph);
+      if (rc || ce != hipSuccess) {
+        (void)hipEventDestroy(ev);
+        if (graph) (void)hipGraphDestroy(graph);
+        return qm_fail(ctx, QM_EHIP, "jacobi_batch: graph capture failed: %s", hipGetErrorString(ce));
+      }
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        (void)hipEventDestroy(ev);
+        return qm_fail(ctx, QM_EHIP, "jacobi_batch: graph instantiation failed: %s", hipGetErrorString(ie));
+      }
+      if (ctx->jgraph_exec) {   // the replaced graph goes once its replays so far have run
+        hipEvent_t old;
+        QM_CHECK_HIP(ctx, hipEventCreateWithFlags(&old, hipEventDisableTiming));
+        QM_CHECK_HIP(ctx, hipEventRecord(old, gs));
+        ctx->graphs.push_back({ctx->jgraph, ctx->jgraph_exec, old});
+      }
+      ctx->jgraph = graph;
+      ctx->jgraph_exec = exec;
+      ctx->jgraph_key.swap(key);
     }
-    for (int64_t r = 0; r < full; ++r) QM_CHECK_HIP(ctx, hipGraphLaunch(exec, gs));
+    for (int64_t r = 0; r < full; ++r) QM_CHECK_HIP(ctx, hipGraphLaunch(ctx->jgraph_exec, gs));
     if (rest) {
       if ((rc = enqueue_block(gs, rest))) return qm_fail(ctx, rc, "jacobi_batch: launch failed");
     }
     QM_CHECK_HIP(ctx, hipEventRecord(ev, gs));
     QM_CHECK_HIP(ctx, hipStreamWaitEvent(s, ev, 0));
-    ctx->graphs.push_back({graph, exec, ev});   // destroyed once ev has fired
+    ctx->graphs.push_back({nullptr, nullptr, ev});   // the event is destroyed once it has fired
   } else {
     for (int64_t r = 0; r < full; ++r)
       if ((rc = enqueue_block(s, block))) return qm_fail(ctx, rc, "jacobi_batch: launch failed");

@@ -100,6 +100,15 @@ struct qm_ctx {
   bool prof = false;
   hipStream_t graph_stream = nullptr;   // non-blocking stream for graph capture (default priority)
   std::vector<LiveGraph> graphs;        // graphs whose replays may still run
+  // the last batched Jacobi block graph, reused while a call's item table (and so every kernel
+  // argument and grid) is the same: a server lane replays one set of molecules round after round
+  std::vector<char> jgraph_key;
+  hipGraph_t jgraph = nullptr;
+  hipGraphExec_t jgraph_exec = nullptr;
+  // pinned staging of batched item tables: the upload is stream-ordered, no host wait for the stream
+  void* table_pinned = nullptr;
+  size_t table_pinned_bytes = 0;
+  hipEvent_t table_uploaded = nullptr;  // recorded after the last upload from table_pinned
   int split_tm = 3;                     // split contraction variant (0..3 tiled, 4 fused, 64 / 128 all-columns)
   int f64_variant = 4;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled, 3 rows, 4 tiled + DMA
   int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)

@@ -73,13 +73,13 @@ def density_batch(mols, grids, precision="mixed", lane=None, normalize=True):
     return outs
 
 
-def hartree_jacobi_batch(mols, grids, densities, sweeps=2000, precision="mixed", boundary="multipole", lane=None,
-                         use_graph=True):
-    """V = -phi/(4 pi) after `sweeps` Jacobi sweeps for every molecule (FP64 device arrays)."""
+def jacobi_prepare(mols, grids, densities, precision="mixed", boundary="multipole", lane=None):
+    """Per-molecule Jacobi state on the lane's stream: rho = seed - density, V holding the boundary
+    (zero or multipole) and, in FP64, the sweep partner.  Returns one (rho, v, tmp) per molecule."""
     lane = lane or Lane(default=True)
     t = _device.torch()
     dev = _device.require()
-    rhos, vs, tmps = [], [], []
+    states = []
     with t.cuda.stream(lane.stream):
         for m, g, q in zip(mols, grids, densities):
             gg = nat.geom(g)
@@ -93,21 +93,39 @@ def hartree_jacobi_batch(mols, grids, densities, sweeps=2000, precision="mixed",
                              nat.dptr(rho), nat.dptr(v), None)
             elif boundary != "zero":
                 raise ValueError("boundary must be 'zero' or 'multipole'")
-            rhos.append(rho)
-            vs.append(v)
-            tmps.append(v.clone() if precision == "fp64" else None)
-    n = len(vs)
-    _, geoms = _arrays(mols, grids)
-    rp = (ctypes.c_void_p * n)(*[r.data_ptr() for r in rhos])
-    vp = (ctypes.c_void_p * n)(*[v.data_ptr() for v in vs])
-    tp = (ctypes.c_void_p * n)(*[(x.data_ptr() if x is not None else 0) for x in tmps])
+            states.append((rho, v, v.clone() if precision == "fp64" else None))
+    return states
+
+
+def jacobi_block(precision):
+    """Sweeps per replayed block: an FP64 block ends with the result in V, an FP32 block starts from
+    the FP64 residual of V and ends by adding its correction to V, so V is the whole state between
+    blocks and sweeps run in any number of calls of whole blocks give the same bits as one call."""
+    return 100 if precision == "fp64" else 50
+
+
+def jacobi_sweeps(grids, states, sweeps, precision="mixed", lane=None, use_graph=True):
+    """`sweeps` Jacobi sweeps on every (rho, v, tmp) state, in one batched launch sequence."""
+    lane = lane or Lane(default=True)
+    n = len(states)
+    geoms = (nat.Geom * n)(*[nat.geom(g) for g in grids])
+    rp = (ctypes.c_void_p * n)(*[st[0].data_ptr() for st in states])
+    vp = (ctypes.c_void_p * n)(*[st[1].data_ptr() for st in states])
+    tp = (ctypes.c_void_p * n)(*[(st[2].data_ptr() if st[2] is not None else 0) for st in states])
     prec = nat.QM_PREC_FP64 if precision == "fp64" else nat.QM_PREC_FP32
-    block = 100 if precision == "fp64" else 50
-    nat.call_ctx(lane.ctx, "qm_jacobi_batch", lane.handle, n, geoms, rp, vp, tp, prec, int(sweeps), block,
-                 1 if use_graph else 0)
+    nat.call_ctx(lane.ctx, "qm_jacobi_batch", lane.handle, n, geoms, rp, vp, tp, prec, int(sweeps),
+                 jacobi_block(precision), 1 if use_graph else 0)
+
+
+def hartree_jacobi_batch(mols, grids, densities, sweeps=2000, precision="mixed", boundary="multipole", lane=None,
+                         use_graph=True):
+    """V = -phi/(4 pi) after `sweeps` Jacobi sweeps for every molecule (FP64 device arrays)."""
+    lane = lane or Lane(default=True)
+    states = jacobi_prepare(mols, grids, densities, precision, boundary, lane)
+    jacobi_sweeps(grids, states, sweeps, precision, lane, use_graph)
     # keep the inputs alive until the stream has consumed them
-    lane._keep = (rhos, tmps)
-    return vs
+    lane._keep = [(st[0], st[2]) for st in states]
+    return [st[1] for st in states]
 
 
 def hartree_fft_batch(mols, grids, densities, precision="mixed", mode="freespace", sigma=0.2, cutoff=6.0,

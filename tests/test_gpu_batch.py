@@ -1,5 +1,7 @@
 """Batched density / Jacobi and the request loop on the GPU: every molecule's
 result in a batch equals its result alone; graph replays equal eager runs."""
+import time
+
 import numpy
 import pytest
 from conftest import BASIS
@@ -128,3 +130,28 @@ def test_server_results_same_alone_and_in_mix(items):
         numpy.testing.assert_array_equal(r.potential, vs[0].cpu().numpy())
     st = srv.stats()
     assert st["requests"] == len(reqs) and st["failed"] == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_server_rounds_late_joiners_equal_alone(items, precision):
+    """Jacobi lanes in rounds of 100 sweeps: three requests submitted after the first was admitted
+    join its lane at a round boundary, and every request's density and potential equal those of
+    the molecule run alone to the same sweep count, bit for bit."""
+    from qmtools import batch
+    from qmtools.serve import Server
+    _say("server rounds", precision)
+    srv = Server(nlanes=1, precision=precision, max_batch=4, sweeps=1000, round_sweeps=100).start()
+    first = srv.submit(*items[3])
+    while first.t_admit is None:
+        time.sleep(1e-3)
+    late = [srv.submit(m, g) for m, g in items[:3]]
+    reqs = [first] + late
+    for r in reqs:
+        r.wait(120)
+    srv.stop()
+    assert srv.joined_running >= 1
+    assert min(r.t_admit for r in late) < first.t_done
+    for r in reqs:
+        qs, vs = batch.run([r.mol], [r.grid], precision, 1000)
+        numpy.testing.assert_array_equal(r.density, qs[0].cpu().numpy())
+        numpy.testing.assert_array_equal(r.potential, vs[0].cpu().numpy())

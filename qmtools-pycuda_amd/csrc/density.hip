@@ -747,8 +747,11 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
   if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
 }
 
-// Tiled FP64 contraction fed by LDS DMA (variant 4): tiles, waves, MFMA order
-// and row-dot of tiled_f64_body; both operands go global -> LDS by
+// Tiled FP64 contraction fed by LDS DMA (variant 4): the 128 x 128 workgroup
+// tiles and per-tile MFMA order of tiled_f64_body, with waves of 16 rows x all
+// 128 columns (so that every wave skips the same tiles of the triangle and the
+// diagonal stages have compile-time tiles, with no runtime tile selection and no
+// cross-wave row-dot reduction); both operands go global -> LDS by
 // global_load_lds_dwordx4 (lane i of a wave brings points / columns 2i, 2i+1 of
 // one k row, so a wave fills one padded LDS row) into a two-stage ring, issued
 // right after the barrier that opens the previous stage.  The loads come from
@@ -761,15 +764,11 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
                                                    const double* __restrict__ M, int np, int nbf, int ncb,
                                                    double* __restrict__ part, int ldp) {
   static_assert(KS % 4 == 0 && (2 * KS) % 8 == 0, "stages of whole MFMA k-steps, rows split over 8 waves");
-  static_assert(KS == 16, "the row-dot below takes one 16-column tile per stage");
+  static_assert(KS == 16, "the diagonal stages below are the block's 8 tiles of 16 columns");
   constexpr int RPW = 2 * KS / 8;   // rows of one operand fetched per wave and stage
   __shared__ __align__(16) double ring[NB][2][KS][kTT + kTPad];
-  __shared__ double red[2][kTT];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  // waves w and w + 4 (one SIMD when waves are dealt round-robin) take the two column halves,
-  // so the MFMAs skipped below are spread evenly over the SIMDs
-  const int wr = wave & 3, wc = wave >> 2;
   const int li = lane & 15, lk = lane >> 4;
   // Column blocks are right-aligned on ne = nbf rounded up to 16: block cb covers columns
   // c0 .. c0 + 127 with c0 = 128 cb - shift, so only block 0 (the cheapest, k < c0 + 128) holds
@@ -777,7 +776,7 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
   const int ne = (nbf + 15) / 16 * 16;
   const int row0 = rt * kTT, c0 = cb * kTT - (ncb * kTT - ne);
   const int nst = (c0 + kTT) / KS;
-  const int cpad = min(max(-(c0 + 64 * wc) / 16, 0), 4);   // tile columns of this wave below column 0
+  const int jd = c0 / KS;   // c0 is a multiple of 16: stage jd + j is the block's diagonal stage j
   // wave w brings k rows RPW (w % 4) .. RPW (w % 4) + RPW - 1 of operand w / 4 (0: basis values, 1: M)
   const int op = wave >> 2, k0 = RPW * (wave & 3);
   const double* src = op == 0 ? phi + row0 + 2 * lane : M + min(max(c0 + 2 * lane, 0), np - 2);   // clamped: unused
@@ -789,79 +788,74 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
       lds_dma16(src + (int64_t)(st * KS + k) * lds_ld, &ring[st % NB][op][k][0]);
     }
   };
-  d4 acc[2][4];
-  double pr[2] = {0.0, 0.0};   // this lane's row-dot partial sums (rows wr 32 + 16 r + li)
+  // Wave w owns rows 16 w .. 16 w + 15 and all 8 tiles of 16 columns of the block, so every wave
+  // drops the same tiles out of the triangle and completes the same tile in each diagonal stage.
+  d4 acc[8];
+  double pr = 0.0;   // this lane's row-dot partial sum (row 16 w + li)
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = d4{0, 0, 0, 0};
+  for (int c = 0; c < 8; ++c) acc[c] = d4{0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < NB - 1; ++q)
     if (q < nst) issue(q);
-  for (int st = 0; st < nst; ++st) {
-    // this thread's loads of stage st are done once at most RPW (NB - 2) newer ones are pending
+  // opens stage st: this thread's loads of st are done once at most RPW (NB - 2) newer ones are
+  // pending, everyone's after the barrier, and everyone is done with st - 1, whose slot is refilled
+  auto open = [&](int st) __attribute__((always_inline)) {
     const int newer = min(NB - 2, nst - 1 - st);
     if (RPW == 2 && newer >= 2) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
     else if (RPW == 2 && newer == 1) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
     else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();   // everyone's loads of st; everyone done with st - 1, whose slot is refilled next
+    __syncthreads();
     if (st + NB - 1 < nst) issue(st + NB - 1);
-    const int buf = st % NB;
-    // M is zero below its diagonal: tile column c (columns n0 = c0 + 64 wc + 16 c ..) needs stage
-    // st only while KS st <= n0 + 15; tile columns below column 0 are never needed.  The first
-    // needed tile column is uniform per wave; each case is its own straight-line body so the
-    // accumulators stay in place.
-    auto body = [&](auto first) __attribute__((always_inline)) {
-      constexpr int cs = decltype(first)::value;
+  };
+  // MFMAs of one stage for the tiles first .. 7 (transposed tiles: M first)
+  auto body = [&](int buf, auto first) __attribute__((always_inline)) {
+    constexpr int cs = decltype(first)::value;
 #pragma unroll
-      for (int ks = 0; ks < KS; ks += 4) {
-        double a[2], b[4];
+    for (int ks = 0; ks < KS; ks += 4) {
+      double b[8];
+      const double a = ring[buf][0][ks + lk][16 * wave + li];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) a[r] = ring[buf][0][ks + lk][wr * 32 + r * 16 + li];
+      for (int c = cs; c < 8; ++c) b[c] = ring[buf][1][ks + lk][c * 16 + li];
 #pragma unroll
-        for (int c = cs; c < 4; ++c) b[c] = ring[buf][1][ks + lk][wc * 64 + c * 16 + li];
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-          for (int c = cs; c < 4; ++c)
-            acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c], a[r], acc[r][c], 0, 0, 0);
-      }
-    };
-    // first needed tile column: KS st <= c0 + 64 wc + 16 c + 15  <=>  c >= (KS st - c0 - 64 wc - 15) / 16
-    const int need = KS * st - c0 - 64 * wc - 15;
-    const int cs = max(min(need > 0 ? (need + 15) / 16 : 0, 4), cpad);
-    // (a tile computed anyway only adds products with M's zeros, or, below column 0, results
-    // the row-dot ignores, so cs is rounded down)
-    if (cs < 2) body(std::integral_constant<int, 0>());
-    else if (cs < 4) body(std::integral_constant<int, 2>());
-    // Row-dot of the tile this stage completes: tile c (columns n0 = c0 + 64 wc + 16 c ..) gets its
-    // last products in the stage with KS st = n0, whose A operand in the ring slot holds the basis
-    // values of exactly its columns (k = n - n0 = lk + 4 q for lane (li, lk), register q).  Tiles
-    // are summed in column order, as f64_rowdot does from global memory.
-    auto rowdot = [&](auto tile) __attribute__((always_inline)) {
-      constexpr int c = decltype(tile)::value;
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) pr[r] = fma(acc[r][c][q], ring[buf][0][lk + 4 * q][wr * 32 + r * 16 + li], pr[r]);
-    };
-    const int cfin = (KS * st - c0 - 64 * wc) / 16;   // exact: c0 is a multiple of 16
-    if (cfin >= cpad) {                                // tiles below column 0 are never summed
-      if (cfin == 0) rowdot(std::integral_constant<int, 0>());
-      else if (cfin == 1) rowdot(std::integral_constant<int, 1>());
-      else if (cfin == 2) rowdot(std::integral_constant<int, 2>());
-      else if (cfin == 3) rowdot(std::integral_constant<int, 3>());
+      for (int c = cs; c < 8; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c], a, acc[c], 0, 0, 0);
     }
-  }
+  };
+  // Row-dot of the tile a diagonal stage completes: tile c (columns n0 = c0 + 16 c ..) gets its last
+  // products in the stage with KS st = n0, whose A operand in the ring slot holds the basis values of
+  // exactly its columns (k = n - n0 = lk + 4 q for lane (li, lk), register q).  Tiles are summed in
+  // column order.
+  auto rowdot = [&](int buf, auto tile) __attribute__((always_inline)) {
+    constexpr int c = decltype(tile)::value;
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    double p = pr[r];
-    p += __shfl_xor(p, 16);
-    p += __shfl_xor(p, 32);
-    if (lk == 0) red[wc][wr * 32 + r * 16 + li] = p;
+    for (int q = 0; q < 4; ++q) pr = fma(acc[c][q], ring[buf][0][lk + 4 * q][16 * wave + li], pr);
+  };
+  // stages below the block's diagonal (none in block 0): every tile, no row-dot
+  for (int st = 0; st < jd; ++st) {
+    open(st);
+    body(st % NB, std::integral_constant<int, 0>());
   }
-  __syncthreads();
-  if (t < kTT) part[(int64_t)cb * ldp + row0 + t] = red[0][t] + red[1][t];
+  // The eight diagonal stages.  M is zero below its diagonal: in diagonal stage j tile c still needs
+  // products while KS j <= 16 c + 15, i.e. c >= j, and tile j gets its last ones.  Block 0 starts at
+  // stage 0 = diagonal stage -jd; its tiles below column 0 are those below -jd, skipped too.
+  auto diag = [&](auto jc) __attribute__((always_inline)) {
+    constexpr int J = decltype(jc)::value;
+    const int st = jd + J;
+    if (st < 0) return;
+    open(st);
+    body(st % NB, jc);
+    rowdot(st % NB, jc);
+  };
+  diag(std::integral_constant<int, 0>());
+  diag(std::integral_constant<int, 1>());
+  diag(std::integral_constant<int, 2>());
+  diag(std::integral_constant<int, 3>());
+  diag(std::integral_constant<int, 4>());
+  diag(std::integral_constant<int, 5>());
+  diag(std::integral_constant<int, 6>());
+  diag(std::integral_constant<int, 7>());
+  pr += __shfl_xor(pr, 16);
+  pr += __shfl_xor(pr, 32);
+  if (lk == 0) part[(int64_t)cb * ldp + row0 + 16 * wave + li] = pr;
 }
 
 // Tiled 16-bit split contraction: the tiling, triangle, workgroup order and

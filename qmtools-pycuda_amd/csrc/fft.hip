@@ -70,11 +70,29 @@ __host__ __device__ __forceinline__ int line_slots(int n) {
 template <typename R>
 __host__ __device__ __forceinline__ int table_slots(int n) { return n + (n >> Skew<R>::S) + 1; }
 
+// Division by a run-time divisor 1 <= d < 2^16 of dividends x < 2^16: q = umulhi(x, ceil(2^32 / d))
+// is exact (the product overshoots x / d by less than x / 2^32 < 2^-16 <= 1 / d, which cannot reach
+// the next integer since frac(x / d) <= 1 - 1 / d); d = 1 is stored as 0.  The mixed-radix passes
+// divide every element and butterfly index by the length, the lines per workgroup, the butterfly
+// count and the span; these take one multiply instead of a division sequence.
+struct FastDiv {
+  uint32_t mul;
+  __host__ static FastDiv of(int d) {
+    FastDiv f;
+    f.mul = d <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d);
+    return f;
+  }
+};
+__device__ __forceinline__ int fdiv(int x, FastDiv f) { return f.mul ? (int)__umulhi((uint32_t)x, f.mul) : x; }
+
 struct AxisPlan {
   int n;              // transform length
   int nstage;
   int radix[kMaxStages];
   int span[kMaxStages];
+  FastDiv ndiv;                 // by n
+  FastDiv mdiv[kMaxStages];     // by n / radix (butterflies per line)
+  FastDiv pdiv[kMaxStages];     // by the span
 };
 
 struct AxisLines {
@@ -82,6 +100,7 @@ struct AxisLines {
   int nl0, nl1;       // lines: l0 in [0, nl0), l1 in [0, nl1)
   int64_t s0, s1;     // line base = l0 * s0 + l1 * s1
   int B;              // lines per workgroup (consecutive l1)
+  FastDiv bdiv;       // by B
 };
 
 // R-point DFT of x (forward sign through the table; the inverse uses a conjugated table).
@@ -146,7 +165,8 @@ __device__ __forceinline__ void dft(Cx<R>* x, const Cx<R>* __restrict__ tw, int 
 // POW2: n, p and RR are powers of two, so the line / butterfly / twiddle indices come from
 // shifts and masks instead of integer divisions by run-time values.
 template <typename R, int RR, bool POW2 = false>
-__device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restrict__ tw, int n, int B, int p) {
+__device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restrict__ tw, int n, int B, int p,
+                                              FastDiv mdiv = FastDiv{0}, FastDiv pdiv = FastDiv{0}) {
   const int LS = line_slots<R>(n);
   constexpr int MAXB = (FftPts<R, POW2>::kMax / kFftThreads + RR - 1) / RR;   // butterflies per thread (B n <= kMax)
   const int m = n / RR;
@@ -158,10 +178,10 @@ __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restric
       l = bf >> lm;
       i = bf & (m - 1);
       k = i & (p - 1);
-    } else {
-      l = bf / m;
+    } else {   // bf < B n <= 4096
+      l = fdiv(bf, mdiv);
       i = bf - l * m;
-      k = i % p;
+      k = i - fdiv(i, pdiv) * p;
     }
   };
   Cx<R> x[MAXB][RR];
@@ -175,7 +195,8 @@ __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restric
 #pragma unroll
       for (int j = 0; j < RR; ++j) {
         Cx<R> v = src[sk<R>(i + j * m)];
-        if (k) v = cmul(v, tw[sk<R>(POW2 ? (j * k * tstep) & (n - 1) : (j * k * tstep) % n)]);
+        // j k tstep <= (RR - 1)(p - 1) n / (p RR) < n: the twiddle index needs no reduction
+        if (k) v = cmul(v, tw[sk<R>(j * k * tstep)]);
         x[b][j] = v;
       }
       dft<R, RR>(x[b], tw, m);
@@ -253,14 +274,13 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
   // line-minor otherwise, so a load instruction reads B adjacent lines); POW2 plans have B a
   // power of two as well, and the split is shifts and masks
   const int lgn = POW2 ? __builtin_ctz(n) : 0, lgb = POW2 ? __builtin_ctz(B) : 0;
-  auto split = [&](int e, int& l, int& a) __attribute__((always_inline)) {
+  auto split = [&](int e) __attribute__((always_inline)) {   // (line, position); e < B n <= 8192
     if (ln.stride == 1) {
-      l = POW2 ? e >> lgn : e / n;
-      a = e - l * n;
-    } else {
-      l = POW2 ? e & (B - 1) : e % B;
-      a = POW2 ? e >> lgb : e / B;
+      const int l = POW2 ? e >> lgn : fdiv(e, plan.ndiv);
+      return make_int2(l, e - l * n);
     }
+    const int a = POW2 ? e >> lgb : fdiv(e, ln.bdiv);
+    return make_int2(POW2 ? e & (B - 1) : e - a * B, a);
   };
   // all of a thread's loads are issued before any is used (B n <= kMax: 8 or 16 per thread)
   constexpr int kPer = FftPts<R, POW2>::kMax / kFftThreads;
@@ -270,18 +290,16 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
     const int e = threadIdx.x + q * kFftThreads;
     ld[q] = Cx<R>{0, 0};
     if (e < B * n) {
-      int l, a;
-      split(e, l, a);
-      if (l < nb) ld[q] = data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride];
+      const int2 la = split(e);
+      if (la.x < nb) ld[q] = data[base0 + (int64_t)la.x * ln.s1 + (int64_t)la.y * ln.stride];
     }
   }
 #pragma unroll
   for (int q = 0; q < kPer; ++q) {
     const int e = threadIdx.x + q * kFftThreads;
     if (e < B * n) {
-      int l, a;
-      split(e, l, a);
-      buf0[l * LS + sk<R>(a)] = ld[q];
+      const int2 la = split(e);
+      buf0[la.x * LS + sk<R>(la.y)] = ld[q];
     }
   }
   __syncthreads();
@@ -298,13 +316,14 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
   } else if (inplace) {
     for (int s = 0; s < plan.nstage; ++s) {
       const int p = plan.span[s];
+      const FastDiv md = plan.mdiv[s], pd = plan.pdiv[s];
       switch (plan.radix[s]) {
-        case 2: stage_inplace<R, 2>(buf0, tw, n, B, p); break;
-        case 3: stage_inplace<R, 3>(buf0, tw, n, B, p); break;
-        case 4: stage_inplace<R, 4>(buf0, tw, n, B, p); break;
-        case 5: stage_inplace<R, 5>(buf0, tw, n, B, p); break;
-        case 7: stage_inplace<R, 7>(buf0, tw, n, B, p); break;
-        default: stage_inplace<R, 8>(buf0, tw, n, B, p); break;
+        case 2: stage_inplace<R, 2>(buf0, tw, n, B, p, md, pd); break;
+        case 3: stage_inplace<R, 3>(buf0, tw, n, B, p, md, pd); break;
+        case 4: stage_inplace<R, 4>(buf0, tw, n, B, p, md, pd); break;
+        case 5: stage_inplace<R, 5>(buf0, tw, n, B, p, md, pd); break;
+        case 7: stage_inplace<R, 7>(buf0, tw, n, B, p, md, pd); break;
+        default: stage_inplace<R, 8>(buf0, tw, n, B, p, md, pd); break;
       }
     }
   } else {
@@ -319,9 +338,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
     res = in;
   }
   for (int e = threadIdx.x; e < B * n; e += kFftThreads) {
-    int l, a;
-    split(e, l, a);
-    if (l < nb) data[base0 + (int64_t)l * ln.s1 + (int64_t)a * ln.stride] = res[l * LS + sk<R>(a)];
+    const int2 la = split(e);
+    if (la.x < nb) data[base0 + (int64_t)la.x * ln.s1 + (int64_t)la.y * ln.stride] = res[la.x * LS + sk<R>(la.y)];
   }
 }
 
@@ -516,9 +534,12 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
   QM_REQUIRE(ctx, (int)f.size() <= kMaxStages, QM_ELIMIT, "fft: too many stages for length %d", n);
   int span = 1;
   plan.nstage = (int)f.size();
+  plan.ndiv = FastDiv::of(n);
   for (size_t q = 0; q < f.size(); ++q) {
     plan.radix[q] = f[q];
     plan.span[q] = span;
+    plan.mdiv[q] = FastDiv::of(n / f[q]);
+    plan.pdiv[q] = FastDiv::of(span);
     span *= f[q];
   }
   const int64_t st[3] = {(int64_t)m[1] * m[2], (int64_t)m[2], 1};
@@ -543,6 +564,7 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
   if (pow2)
     while (B & (B - 1)) B &= B - 1;   // the power-of-two kernel splits indices by shifts
   ln.B = B;
+  ln.bdiv = FastDiv::of(B);
   const int lines_per_row = (ln.nl1 + B - 1) / B;
   const int64_t nwg = (int64_t)ln.nl0 * lines_per_row;
   const size_t shmem = sizeof(Cx<R>) * ((size_t)table_slots<R>(n) + (size_t)planes * B * line_slots<R>(n));

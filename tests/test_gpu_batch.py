@@ -155,3 +155,32 @@ def test_server_rounds_late_joiners_equal_alone(items, precision):
         qs, vs = batch.run([r.mol], [r.grid], precision, 1000)
         numpy.testing.assert_array_equal(r.density, qs[0].cpu().numpy())
         numpy.testing.assert_array_equal(r.potential, vs[0].cpu().numpy())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_jacobi_rounds_alternating_sets_equal_one_call(items, precision):
+    """The batched Jacobi keeps one block graph per context and reuses it while the item table is
+    unchanged: sweeping set A, then set B, then A again (each in calls of whole blocks) must give
+    every molecule the bits of one call with all its sweeps."""
+    from qmtools import batch
+    _say("jacobi alternating sets", precision)
+    lane = batch.Lane()
+    blk = batch.jacobi_block(precision)
+    mols = [m for m, _ in items]
+    grids = [g for _, g in items]
+    qs = batch.density_batch(mols, grids, precision, lane)
+    sets = {"A": [0, 1], "B": [2, 3, 0]}
+    states = {k: batch.jacobi_prepare([mols[i] for i in v], [grids[i] for i in v], [qs[i] for i in v],
+                                      precision, "multipole", lane) for k, v in sets.items()}
+    done = {"A": 0, "B": 0}
+    for k in ("A", "B", "A", "B", "A"):
+        batch.jacobi_sweeps([grids[i] for i in sets[k]], states[k], 2 * blk, precision, lane)
+        done[k] += 2 * blk
+    lane.stream.synchronize()
+    for k, v in sets.items():
+        want = batch.hartree_jacobi_batch([mols[i] for i in v], [grids[i] for i in v], [qs[i] for i in v],
+                                          done[k], precision, "multipole", lane)
+        lane.stream.synchronize()
+        for st, w in zip(states[k], want):
+            numpy.testing.assert_array_equal(st[1].cpu().numpy(), w.cpu().numpy())
+    lane.close()

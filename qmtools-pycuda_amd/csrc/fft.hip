@@ -219,25 +219,30 @@ __device__ __forceinline__ void stage_inplace(Cx<R>* buf, const Cx<R>* __restric
 }
 
 // Any radix, out of place between two LDS planes (used when a length has a prime factor > 7).
+// Indices as in stage_inplace (quotients by multiply-high, twiddle index below n); the DFT table
+// index (j q mod RR) m is carried from one q to the next.
 template <typename R>
 __device__ __forceinline__ void stage_generic(const Cx<R>* in, Cx<R>* out, const Cx<R>* __restrict__ tw, int n,
-                                              int B, int RR, int p) {
+                                              int B, int RR, int p, FastDiv mdiv, FastDiv pdiv) {
   const int m = n / RR;
   const int tstep = n / (p * RR);
   const int LS = line_slots<R>(n);
   for (int bf = threadIdx.x; bf < B * m; bf += kFftThreads) {
-    const int l = bf / m, i = bf - (bf / m) * m, k = i % p;
+    const int l = fdiv(bf, mdiv), i = bf - l * m, k = i - fdiv(i, pdiv) * p;
     const Cx<R>* src = in + l * LS;
     Cx<R>* dst = out + l * LS;
     const int o = (i - k) * RR + k;
     for (int j = 0; j < RR; ++j) {
       Cx<R> acc{0, 0};
+      int jq = 0;   // j q mod RR
       for (int q = 0; q < RR; ++q) {
         Cx<R> v = src[sk<R>(i + q * m)];
-        if (k) v = cmul(v, tw[sk<R>((q * k * tstep) % n)]);
-        const Cx<R> t = cmul(v, tw[sk<R>(((j * q) % RR) * m)]);
+        if (k) v = cmul(v, tw[sk<R>(q * k * tstep)]);
+        const Cx<R> t = cmul(v, tw[sk<R>(jq * m)]);
         acc.x += t.x;
         acc.y += t.y;
+        jq += j;
+        if (jq >= RR) jq -= RR;
       }
       dst[sk<R>(o + j * p)] = acc;
     }
@@ -330,7 +335,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
     Cx<R>* in = buf0;
     Cx<R>* out = buf1;
     for (int s = 0; s < plan.nstage; ++s) {
-      stage_generic<R>(in, out, tw, n, B, plan.radix[s], plan.span[s]);
+      stage_generic<R>(in, out, tw, n, B, plan.radix[s], plan.span[s], plan.mdiv[s], plan.pdiv[s]);
       Cx<R>* t = in;
       in = out;
       out = t;
@@ -411,17 +416,17 @@ __global__ void k_mul_real(Cx<R>* __restrict__ a, const R* __restrict__ gh, int6
   a[idx] = v;
 }
 
-// g(d) on the 2n cyclic grid, C order; d_i = i h for i <= n, (i - 2n) h otherwise
+// g(d) on the m0 x m1 x m2 cyclic grid (m >= 2n), C order; |d_i| = min(i, m - i) h.  Distances
+// between two voxels of the grid are below n, so only |d| < n is ever used.
 template <typename R>
-__global__ void k_green(Cx<R>* __restrict__ a, int n0, int n1, int n2, double h_ang) {
-  const int m0 = 2 * n0, m1 = 2 * n1, m2 = 2 * n2;
+__global__ void k_green(Cx<R>* __restrict__ a, int m0, int m1, int m2, double h_ang) {
   const int64_t N = (int64_t)m0 * m1 * m2;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= N) return;
   const int k = (int)(idx % m2), j = (int)((idx / m2) % m1), i = (int)(idx / ((int64_t)m1 * m2));
-  const double x = (i <= n0 ? i : i - m0) * h_ang;
-  const double y = (j <= n1 ? j : j - m1) * h_ang;
-  const double z = (k <= n2 ? k : k - m2) * h_ang;
+  const double x = min(i, m0 - i) * h_ang;
+  const double y = min(j, m1 - j) * h_ang;
+  const double z = min(k, m2 - k) * h_ang;
   const double r = sqrt(x * x + y * y + z * z);
   const double c0 = 2.0 * (3.0 * asinh(1.0 / sqrt(2.0)) - M_PI / 4.0);
   const double g = (idx == 0) ? c0 / h_ang : 1.0 / r;
@@ -520,6 +525,18 @@ int twiddle_table(qm_ctx* ctx, int n, const Cx<R>** out) {
   return QM_OK;
 }
 
+// Free-space padding: the smallest length >= 2n with no prime factor above 7.  A cyclic length of
+// at least 2n - 1 holds the linear convolution of two n-point signals exactly, and such a length
+// keeps every pass on the in-place radices (2n = 304 = 16 x 19 would take the generic stage; 315 = 3^2 5 7).
+int freespace_len(int n) {
+  for (int v = 2 * n;; ++v) {
+    int x = v;
+    for (int r : {2, 3, 5, 7})
+      while (x % r == 0) x /= r;
+    if (x == 1) return v;
+  }
+}
+
 // one pass along `axis` of a C-order [m0][m1][m2] buffer, on the lines whose
 // other two indices are below (lim_a, lim_b) (in increasing axis order)
 template <typename R, bool INV>
@@ -601,11 +618,11 @@ int green_hat(qm_ctx* ctx, hipStream_t s, const qm_geom* g, Cx<R>* work, const R
       *out = (const R*)gr.gh;
       return QM_OK;
     }
-  const int m[3] = {2 * g->nx, 2 * g->ny, 2 * g->nz};
+  const int m[3] = {freespace_len(g->nx), freespace_len(g->ny), freespace_len(g->nz)};
   const int64_t M = (int64_t)m[0] * m[1] * m[2];
   const double h_ang = g->step / kAng2Bor;
   const unsigned blocks = (unsigned)((M + 255) / 256);
-  hipLaunchKernelGGL(k_green<R>, dim3(blocks), dim3(256), 0, s, work, g->nx, g->ny, g->nz, h_ang);
+  hipLaunchKernelGGL(k_green<R>, dim3(blocks), dim3(256), 0, s, work, m[0], m[1], m[2], h_ang);
   QM_CHECK_LAUNCH(ctx);
   int rc;
   if ((rc = fft_pass<R, false>(ctx, s, work, m, 2, m[0], m[1]))) return rc;
@@ -638,7 +655,8 @@ int hartree_fft(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, const
                 double cutoff, int mode, TIO* out) {
   const bool fs = mode == QM_FFT_FREESPACE;
   const int n[3] = {g->nx, g->ny, g->nz};
-  const int m[3] = {fs ? 2 * n[0] : n[0], fs ? 2 * n[1] : n[1], fs ? 2 * n[2] : n[2]};
+  const int m[3] = {fs ? freespace_len(n[0]) : n[0], fs ? freespace_len(n[1]) : n[1],
+                    fs ? freespace_len(n[2]) : n[2]};
   const int64_t M = (int64_t)m[0] * m[1] * m[2];
   Cx<R>* work = nullptr;
   int rc = qm_ws(ctx, s, WS_FFT_A, sizeof(Cx<R>) * (size_t)M, (void**)&work);

@@ -112,17 +112,41 @@ _PHI_BUDGET = 4 << 30          # the native density keeps at most this much basi
 _GREEN_BUDGET = 4 << 30        # and at most this much of cached Green's-function transforms
 
 
+def freespace_len(n):
+    """Cyclic length of a free-space FFT axis of n points: the smallest length >= 2n with no prime
+    factor above 7 (as the native library pads)."""
+    v = 2 * n
+    while True:
+        x = v
+        for r in (2, 3, 5, 7):
+            while x % r == 0:
+                x //= r
+        if x == 1:
+            return v
+        v += 1
+
+
+def _padded_points(req):
+    """Points of the request's padded free-space FFT grid (an upper bound of 27 x npts without a grid
+    shape: each axis pads to less than 3 n)."""
+    shape = getattr(getattr(req, "grid", None), "shape", None)
+    if shape is None or len(shape) < 3:
+        return 27 * req.npts
+    nx, ny, nz = (int(d) for d in shape[-3:])
+    return freespace_len(nx) * freespace_len(ny) * freespace_len(nz)
+
+
 def workspace_estimate(reqs, precision="mixed", poisson="jacobi"):
     """Device bytes a lane's native context grows to while it runs this batch: basis values of one
     round, the FP32 Jacobi correction buffers or the FFT work buffer and Green's-function cache."""
     elt = 8 if precision == "fp64" else 4
     npts = sum(r.npts for r in reqs)
-    biggest = max((r.npts for r in reqs), default=0)
     nbf_pad = max(((int(r.mol.nbf) + 63) // 64 * 64 for r in reqs), default=0)
     phi = min(_PHI_BUDGET, npts * nbf_pad * elt)
     if poisson == "fft":
         cplx = 16 if precision == "fp64" else 8
-        hartree = 8 * biggest * cplx + min(_GREEN_BUDGET, sum(8 * r.npts * elt for r in reqs))
+        padded = [_padded_points(r) for r in reqs]
+        hartree = max(padded, default=0) * cplx + min(_GREEN_BUDGET, sum(m * elt for m in padded))
     else:
         hartree = 0 if precision == "fp64" else 12 * npts
     return phi + hartree

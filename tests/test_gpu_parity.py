@@ -344,9 +344,11 @@ def test_fft_fp64(qm, small, mode):
     assert _rel(v, want) <= 1e-10
 
 
-@pytest.mark.parametrize("shape", [(24, 40, 16), (88, 16, 24)])
+@pytest.mark.parametrize("shape", [(24, 40, 16), (88, 16, 24), (152, 16, 24)])
 def test_fft_mixed_radix_lengths(qm, small, shape):
-    """Lengths with factors 3, 5 (in-place radices) and 11 (generic radix path)."""
+    """Lengths with factors 3, 5 (in-place radices) and 11, 19 (generic radix path, periodic); in free
+    space 2n = 176 and 304 are padded to 180 and 315 (no prime factor above 7), against the oracle's
+    2n-point convolution."""
     from oracle import fft as off
     mol, _, _ = small
     nx, ny, nz = shape

@@ -288,3 +288,25 @@ def test_rounds_prefer_an_idle_lane(basis):
     assert len(lanes[first.rid]) == 1 and len(lanes[second.rid]) == 1
     assert lanes[first.rid] != lanes[second.rid]
     assert srv.joined_running == 0 and srv.max_concurrent == 2
+
+
+def test_freespace_padding_lengths():
+    """The estimate pads like the native FFT: 2n when it has no prime factor above 7, else the next
+    such length (176 = 16 x 11 -> 180, 304 = 16 x 19 -> 315, 368 = 16 x 23 -> 375)."""
+    from qmtools.serve import freespace_len
+    assert [freespace_len(n) for n in (1, 32, 48, 64, 80, 96, 128, 256)] == [2, 64, 96, 128, 160, 192, 256, 512]
+    assert freespace_len(88) == 180 and freespace_len(152) == 315 and freespace_len(184) == 375
+    for n in range(1, 400):
+        v = freespace_len(n)
+        assert v >= 2 * n
+        x = v
+        for r in (2, 3, 5, 7):
+            while x % r == 0:
+                x //= r
+        assert x == 1
+        for w in range(2 * n, v):
+            y = w
+            for r in (2, 3, 5, 7):
+                while y % r == 0:
+                    y //= r
+            assert y != 1

@@ -101,6 +101,20 @@ struct AxisLines {
   int64_t s0, s1;     // line base = l0 * s0 + l1 * s1
   int B;              // lines per workgroup (consecutive l1)
   FastDiv bdiv;       // by B
+  int nin;            // positions >= nin of a line are zero on input (not loaded)
+  int nout;           // positions >= nout are not needed after the pass (not stored)
+};
+
+// Transform of the free-space Green's function for one padded shape, computed at unit grid step:
+// the transform is real and even in every axis, so one octant is kept, o1 = m1 / 2 + 1 and
+// o2 = m2 / 2 + 1 wide; the value at grid step h is g[min(i, m0 - i)][min(j, m1 - j)][min(k, m2 - k)]
+// times scale = h^2 (Angstrom), since g(d) = 1 / |d| scales as 1 / h and the voxel volume as h^3.
+template <typename R>
+struct GreenView {
+  const R* g;
+  int m0, m1, m2;
+  int o1, o2;
+  R scale;
 };
 
 // R-point DFT of x (forward sign through the table; the inverse uses a conjugated table).
@@ -250,13 +264,57 @@ __device__ __forceinline__ void stage_generic(const Cx<R>* in, Cx<R>* out, const
   __syncthreads();
 }
 
+// The stages of one transform of a workgroup's B lines, from buf0 (buf1: the second plane of the
+// generic stages); returns the plane that holds the result.
+template <typename R, bool POW2>
+__device__ __forceinline__ Cx<R>* run_stages(const AxisPlan& plan, Cx<R>* buf0, Cx<R>* buf1, const Cx<R>* tw, int n,
+                                             int B, int inplace) {
+  if (POW2) {
+    for (int s = 0; s < plan.nstage; ++s) {
+      const int p = plan.span[s];
+      switch (plan.radix[s]) {
+        case 2: stage_inplace<R, 2, true>(buf0, tw, n, B, p); break;
+        case 4: stage_inplace<R, 4, true>(buf0, tw, n, B, p); break;
+        default: stage_inplace<R, 8, true>(buf0, tw, n, B, p); break;
+      }
+    }
+    return buf0;
+  }
+  if (inplace) {
+    for (int s = 0; s < plan.nstage; ++s) {
+      const int p = plan.span[s];
+      const FastDiv md = plan.mdiv[s], pd = plan.pdiv[s];
+      switch (plan.radix[s]) {
+        case 2: stage_inplace<R, 2>(buf0, tw, n, B, p, md, pd); break;
+        case 3: stage_inplace<R, 3>(buf0, tw, n, B, p, md, pd); break;
+        case 4: stage_inplace<R, 4>(buf0, tw, n, B, p, md, pd); break;
+        case 5: stage_inplace<R, 5>(buf0, tw, n, B, p, md, pd); break;
+        case 7: stage_inplace<R, 7>(buf0, tw, n, B, p, md, pd); break;
+        default: stage_inplace<R, 8>(buf0, tw, n, B, p, md, pd); break;
+      }
+    }
+    return buf0;
+  }
+  Cx<R>* in = buf0;
+  Cx<R>* out = buf1;
+  for (int s = 0; s < plan.nstage; ++s) {
+    stage_generic<R>(in, out, tw, n, B, plan.radix[s], plan.span[s], plan.mdiv[s], plan.pdiv[s]);
+    Cx<R>* t = in;
+    in = out;
+    out = t;
+  }
+  return in;
+}
+
 // One FFT pass along one axis.  INV: inverse (conjugated) transform, unnormalised.
 // inplace: all radices <= 8 (LDS = table + B lines); otherwise two planes of B lines.
 // POW2: the length is a power of two (radices 8, 4, 2 only), which keeps the
 // register footprint of the butterflies, and so the kernel's, small.
-template <typename R, bool INV, bool POW2>
+// MUL (axis 0 of the free-space solve only): forward transform, product with the Green's
+// function transform, inverse transform, all on the lines held in LDS.
+template <typename R, bool INV, bool POW2, bool MUL = false>
 __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ data, const Cx<R>* __restrict__ twg,
-                                                       AxisPlan plan, AxisLines ln, int inplace) {
+                                                       AxisPlan plan, AxisLines ln, int inplace, GreenView<R> gv) {
   extern __shared__ unsigned char smem[];
   const int n = plan.n;
   const int B = ln.B;
@@ -296,7 +354,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
     ld[q] = Cx<R>{0, 0};
     if (e < B * n) {
       const int2 la = split(e);
-      if (la.x < nb) ld[q] = data[base0 + (int64_t)la.x * ln.s1 + (int64_t)la.y * ln.stride];
+      if (la.x < nb && la.y < ln.nin) ld[q] = data[base0 + (int64_t)la.x * ln.s1 + (int64_t)la.y * ln.stride];
     }
   }
 #pragma unroll
@@ -308,43 +366,28 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(Cx<R>* __restrict__ da
     }
   }
   __syncthreads();
-  Cx<R>* res = buf0;
-  if (POW2) {
-    for (int s = 0; s < plan.nstage; ++s) {
-      const int p = plan.span[s];
-      switch (plan.radix[s]) {
-        case 2: stage_inplace<R, 2, true>(buf0, tw, n, B, p); break;
-        case 4: stage_inplace<R, 4, true>(buf0, tw, n, B, p); break;
-        default: stage_inplace<R, 8, true>(buf0, tw, n, B, p); break;
+  Cx<R>* res = run_stages<R, POW2>(plan, buf0, buf1, tw, n, B, inplace);
+  if (MUL) {
+    // line (j, k) = (l0, l1b + l), position i; the octant index folds each of them
+    const int j = (int)l0, jj = min(j, gv.m1 - j);
+    for (int e = threadIdx.x; e < B * n; e += kFftThreads) {
+      const int2 la = split(e);
+      if (la.x < nb) {
+        const int k = l1b + la.x, i = la.y;
+        const R f = gv.g[((int64_t)min(i, gv.m0 - i) * gv.o1 + jj) * gv.o2 + min(k, gv.m2 - k)] * gv.scale;
+        Cx<R>& v = res[la.x * LS + sk<R>(la.y)];
+        v.x *= f;
+        v.y *= f;
       }
     }
-  } else if (inplace) {
-    for (int s = 0; s < plan.nstage; ++s) {
-      const int p = plan.span[s];
-      const FastDiv md = plan.mdiv[s], pd = plan.pdiv[s];
-      switch (plan.radix[s]) {
-        case 2: stage_inplace<R, 2>(buf0, tw, n, B, p, md, pd); break;
-        case 3: stage_inplace<R, 3>(buf0, tw, n, B, p, md, pd); break;
-        case 4: stage_inplace<R, 4>(buf0, tw, n, B, p, md, pd); break;
-        case 5: stage_inplace<R, 5>(buf0, tw, n, B, p, md, pd); break;
-        case 7: stage_inplace<R, 7>(buf0, tw, n, B, p, md, pd); break;
-        default: stage_inplace<R, 8>(buf0, tw, n, B, p, md, pd); break;
-      }
-    }
-  } else {
-    Cx<R>* in = buf0;
-    Cx<R>* out = buf1;
-    for (int s = 0; s < plan.nstage; ++s) {
-      stage_generic<R>(in, out, tw, n, B, plan.radix[s], plan.span[s], plan.mdiv[s], plan.pdiv[s]);
-      Cx<R>* t = in;
-      in = out;
-      out = t;
-    }
-    res = in;
+    for (int q = threadIdx.x; q < n; q += kFftThreads) tw[sk<R>(q)].y = -tw[sk<R>(q)].y;
+    __syncthreads();
+    res = run_stages<R, POW2>(plan, res, res == buf0 ? buf1 : buf0, tw, n, B, inplace);
   }
   for (int e = threadIdx.x; e < B * n; e += kFftThreads) {
     const int2 la = split(e);
-    if (la.x < nb) data[base0 + (int64_t)la.x * ln.s1 + (int64_t)la.y * ln.stride] = res[la.x * LS + sk<R>(la.y)];
+    if (la.x < nb && la.y < ln.nout)
+      data[base0 + (int64_t)la.x * ln.s1 + (int64_t)la.y * ln.stride] = res[la.x * LS + sk<R>(la.y)];
   }
 }
 
@@ -405,39 +448,32 @@ __global__ void k_poisson_periodic(Cx<R>* __restrict__ a, int n0, int n1, int n2
   a[idx] = v;
 }
 
+// g(d) at unit grid step on the m0 x m1 x m2 cyclic grid (m >= 2n), C order: |d_i| = min(i, m - i),
+// g(0) = C0.  Distances between two voxels of the grid are below n, so only |d| < n is ever used.
 template <typename R>
-__global__ void k_mul_real(Cx<R>* __restrict__ a, const R* __restrict__ gh, int64_t N) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= N) return;
-  const R f = gh[idx];
-  Cx<R> v = a[idx];
-  v.x *= f;
-  v.y *= f;
-  a[idx] = v;
-}
-
-// g(d) on the m0 x m1 x m2 cyclic grid (m >= 2n), C order; |d_i| = min(i, m - i) h.  Distances
-// between two voxels of the grid are below n, so only |d| < n is ever used.
-template <typename R>
-__global__ void k_green(Cx<R>* __restrict__ a, int m0, int m1, int m2, double h_ang) {
+__global__ void k_green(Cx<R>* __restrict__ a, int m0, int m1, int m2) {
   const int64_t N = (int64_t)m0 * m1 * m2;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= N) return;
   const int k = (int)(idx % m2), j = (int)((idx / m2) % m1), i = (int)(idx / ((int64_t)m1 * m2));
-  const double x = min(i, m0 - i) * h_ang;
-  const double y = min(j, m1 - j) * h_ang;
-  const double z = min(k, m2 - k) * h_ang;
+  const double x = min(i, m0 - i);
+  const double y = min(j, m1 - j);
+  const double z = min(k, m2 - k);
   const double r = sqrt(x * x + y * y + z * z);
   const double c0 = 2.0 * (3.0 * asinh(1.0 / sqrt(2.0)) - M_PI / 4.0);
-  const double g = (idx == 0) ? c0 / h_ang : 1.0 / r;
+  const double g = (idx == 0) ? c0 : 1.0 / r;
   a[idx] = Cx<R>{(R)g, (R)0};
 }
 
+// the octant i <= m0 / 2, j <= m1 / 2, k <= m2 / 2 of the transform's real part, times scale
 template <typename R>
-__global__ void k_green_store(const Cx<R>* __restrict__ a, R* __restrict__ gh, int64_t N, double scale) {
+__global__ void k_green_store(const Cx<R>* __restrict__ a, R* __restrict__ gh, int m1, int m2, int o0, int o1,
+                              int o2, double scale) {
+  const int64_t N = (int64_t)o0 * o1 * o2;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= N) return;
-  gh[idx] = (R)((double)a[idx].x * scale);
+  const int k = (int)(idx % o2), j = (int)((idx / o2) % o1), i = (int)(idx / ((int64_t)o1 * o2));
+  gh[idx] = (R)((double)a[((int64_t)i * m1 + j) * m2 + k].x * scale);
 }
 
 template <typename R, typename TO>
@@ -468,17 +504,16 @@ std::vector<int> factorize(int n) {
 
 }  // namespace
 
-// Transforms of the free-space Green's function, one per (grid shape, step, precision), are
-// kept up to kGreenCacheBytes per context; beyond that the least recently used are released
-// (stream-ordered, from the stream-ordered allocator).
+// Transforms of the free-space Green's function at unit step, one octant per (padded shape,
+// precision), are kept up to kGreenCacheBytes per context; beyond that the least recently used
+// are released (stream-ordered, from the stream-ordered allocator).
 constexpr size_t kGreenCacheBytes = (size_t)4 << 30;
 
 struct FftPlanCache {
   std::map<std::pair<int, int>, void*> twiddles;            // (n, bytes per real) -> device table
   size_t twiddle_bytes = 0;
   struct Green {
-    int n0, n1, n2;
-    double h;
+    int m0, m1, m2;   // padded shape
     int rbytes;
     void* gh;
     size_t bytes;
@@ -538,9 +573,12 @@ int freespace_len(int n) {
 }
 
 // one pass along `axis` of a C-order [m0][m1][m2] buffer, on the lines whose
-// other two indices are below (lim_a, lim_b) (in increasing axis order)
+// other two indices are below (lim_a, lim_b) (in increasing axis order); positions >= nin
+// are read as zero and positions >= nout are not written.  gv: forward transform, product
+// with the Green's function transform and inverse transform in one pass (axis 0 only).
 template <typename R, bool INV>
-int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, int lim_a, int lim_b) {
+int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, int lim_a, int lim_b, int nin,
+             int nout, const GreenView<R>* gv = nullptr) {
   const int n = m[axis];
   const Cx<R>* tw = nullptr;
   int rc = twiddle_table<R>(ctx, n, &tw);
@@ -568,6 +606,10 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
   ln.nl1 = lim_b;
   ln.s0 = st[oa];
   ln.s1 = st[ob];
+  ln.nin = nin;
+  ln.nout = nout;
+  QM_REQUIRE(ctx, nin >= 1 && nin <= n && nout >= 1 && nout <= n && (!gv || axis == 0), QM_EINVAL,
+             "fft: bad pass");
   bool inplace = true;
   for (int r : f) inplace = inplace && r <= 8;
   const int planes = inplace ? 1 : 2;
@@ -591,45 +633,51 @@ int fft_pass(qm_ctx* ctx, hipStream_t s, Cx<R>* data, const int m[3], int axis, 
   if (nwg == 0) return QM_OK;
   static bool attr_set = false;
   if (!attr_set) {
-    QM_CHECK_HIP(ctx, hipFuncSetAttribute((const void*)k_fft_axis<R, INV, false>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
-    QM_CHECK_HIP(ctx, hipFuncSetAttribute((const void*)k_fft_axis<R, INV, true>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+    const void* fns[4] = {(const void*)k_fft_axis<R, INV, false>, (const void*)k_fft_axis<R, INV, true>,
+                          (const void*)k_fft_axis<R, false, false, true>, (const void*)k_fft_axis<R, false, true, true>};
+    for (const void* f : fns)
+      QM_CHECK_HIP(ctx, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
     attr_set = true;
   }
   ProfScope ps(ctx, s, QM_K_FFT_AXIS);
-  if (pow2)
-    hipLaunchKernelGGL((k_fft_axis<R, INV, true>), dim3((unsigned)nwg), dim3(kFftThreads), shmem, s, data, tw, plan, ln,
-                       1);
-  else
-    hipLaunchKernelGGL((k_fft_axis<R, INV, false>), dim3((unsigned)nwg), dim3(kFftThreads), shmem, s, data, tw, plan,
-                       ln, inplace ? 1 : 0);
+  const GreenView<R> g0 = gv ? *gv : GreenView<R>{};
+  const dim3 grid((unsigned)nwg), thr(kFftThreads);
+  const int ip = inplace ? 1 : 0;
+  if (gv) {
+    if (pow2)
+      hipLaunchKernelGGL((k_fft_axis<R, false, true, true>), grid, thr, shmem, s, data, tw, plan, ln, 1, g0);
+    else
+      hipLaunchKernelGGL((k_fft_axis<R, false, false, true>), grid, thr, shmem, s, data, tw, plan, ln, ip, g0);
+  } else if (pow2) {
+    hipLaunchKernelGGL((k_fft_axis<R, INV, true>), grid, thr, shmem, s, data, tw, plan, ln, 1, g0);
+  } else {
+    hipLaunchKernelGGL((k_fft_axis<R, INV, false>), grid, thr, shmem, s, data, tw, plan, ln, ip, g0);
+  }
   QM_CHECK_LAUNCH(ctx);
   return QM_OK;
 }
 
 template <typename R>
-int green_hat(qm_ctx* ctx, hipStream_t s, const qm_geom* g, Cx<R>* work, const R** out) {
+int green_hat(qm_ctx* ctx, hipStream_t s, const int m[3], Cx<R>* work, const R** out) {
   if (!ctx->fft) ctx->fft = new FftPlanCache();
   FftPlanCache& fc = *ctx->fft;
   for (auto& gr : fc.greens)
-    if (gr.n0 == g->nx && gr.n1 == g->ny && gr.n2 == g->nz && gr.h == g->step && gr.rbytes == (int)sizeof(R)) {
+    if (gr.m0 == m[0] && gr.m1 == m[1] && gr.m2 == m[2] && gr.rbytes == (int)sizeof(R)) {
       gr.used = ++fc.clock;
       *out = (const R*)gr.gh;
       return QM_OK;
     }
-  const int m[3] = {freespace_len(g->nx), freespace_len(g->ny), freespace_len(g->nz)};
   const int64_t M = (int64_t)m[0] * m[1] * m[2];
-  const double h_ang = g->step / kAng2Bor;
-  const unsigned blocks = (unsigned)((M + 255) / 256);
-  hipLaunchKernelGGL(k_green<R>, dim3(blocks), dim3(256), 0, s, work, m[0], m[1], m[2], h_ang);
+  hipLaunchKernelGGL(k_green<R>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, work, m[0], m[1], m[2]);
   QM_CHECK_LAUNCH(ctx);
   int rc;
-  if ((rc = fft_pass<R, false>(ctx, s, work, m, 2, m[0], m[1]))) return rc;
-  if ((rc = fft_pass<R, false>(ctx, s, work, m, 1, m[0], m[2]))) return rc;
-  if ((rc = fft_pass<R, false>(ctx, s, work, m, 0, m[1], m[2]))) return rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 2, m[0], m[1], m[2], m[2]))) return rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 1, m[0], m[2], m[1], m[1]))) return rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 0, m[1], m[2], m[0], m[0]))) return rc;
+  const int o[3] = {m[0] / 2 + 1, m[1] / 2 + 1, m[2] / 2 + 1};
+  const int64_t O = (int64_t)o[0] * o[1] * o[2];
   // make room: release the least recently used transforms (work queued earlier on s still reads them)
-  const size_t bytes = sizeof(R) * (size_t)M;
+  const size_t bytes = sizeof(R) * (size_t)O;
   while (!fc.greens.empty() && fc.green_bytes + bytes > kGreenCacheBytes) {
     size_t lru = 0;
     for (size_t i = 1; i < fc.greens.size(); ++i)
@@ -640,11 +688,12 @@ int green_hat(qm_ctx* ctx, hipStream_t s, const qm_geom* g, Cx<R>* work, const R
   }
   void* gh = nullptr;
   QM_CHECK_HIP(ctx, hipMallocAsync(&gh, bytes, s));
-  // V = -(h^3 / 4 pi) IFFT(rhs^ g^) / M   (rhs = -rho/eps0, g = 1/r)
-  const double scale = -(h_ang * h_ang * h_ang) / (4.0 * M_PI) / (double)M;
-  hipLaunchKernelGGL(k_green_store<R>, dim3(blocks), dim3(256), 0, s, (const Cx<R>*)work, (R*)gh, M, scale);
+  // V = -(h^3 / 4 pi) IFFT(rhs^ g^) / M   (rhs = -rho/eps0, g = 1/r); the step enters as h^2 at use
+  const double scale = -1.0 / (4.0 * M_PI) / (double)M;
+  hipLaunchKernelGGL(k_green_store<R>, dim3((unsigned)((O + 255) / 256)), dim3(256), 0, s, (const Cx<R>*)work,
+                     (R*)gh, m[1], m[2], o[0], o[1], o[2], scale);
   QM_CHECK_LAUNCH(ctx);
-  fc.greens.push_back({g->nx, g->ny, g->nz, g->step, (int)sizeof(R), gh, bytes, ++fc.clock});
+  fc.greens.push_back({m[0], m[1], m[2], (int)sizeof(R), gh, bytes, ++fc.clock});
   fc.green_bytes += bytes;
   *out = (const R*)gh;
   return QM_OK;
@@ -661,33 +710,41 @@ int hartree_fft(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, const
   Cx<R>* work = nullptr;
   int rc = qm_ws(ctx, s, WS_FFT_A, sizeof(Cx<R>) * (size_t)M, (void**)&work);
   if (rc) return rc;
-  const R* gh = nullptr;
+  GreenView<R> gv{};
   if (fs) {
-    if ((rc = green_hat<R>(ctx, s, g, work, &gh))) return rc;
-    QM_CHECK_HIP(ctx, hipMemsetAsync(work, 0, sizeof(Cx<R>) * (size_t)M, s));
+    if ((rc = green_hat<R>(ctx, s, m, work, &gv.g))) return rc;
+    const double h_ang = g->step / kAng2Bor;
+    gv.m0 = m[0];
+    gv.m1 = m[1];
+    gv.m2 = m[2];
+    gv.o1 = m[1] / 2 + 1;
+    gv.o2 = m[2] / 2 + 1;
+    gv.scale = (R)(h_ang * h_ang);
   }
-  const GeomView gv = geom_view(g);
+  const GeomView gview = geom_view(g);
   const double cut2 = (cutoff * sigma) * (cutoff * sigma);
+  // rhs on [i < n0][j < n1][k < n2]; every pass reads only positions already written (the
+  // padding is never cleared: it is read as zero) and writes only positions read later
   hipLaunchKernelGGL((k_rhs<R, TIO>), dim3((n[0] + 31) / 32, n[1], (n[2] + 31) / 32), dim3(kThreads), 0, s, q,
-                     (const double4*)mol->atoms, mol->natoms, gv, sigma, cut2, m[1], m[2], work);
+                     (const double4*)mol->atoms, mol->natoms, gview, sigma, cut2, m[1], m[2], work);
   QM_CHECK_LAUNCH(ctx);
-  // forward: z on lines (i < n0, j < n1), y on (i < n0, all k), x on all (j, k)
-  if ((rc = fft_pass<R, false>(ctx, s, work, m, 2, n[0], n[1]))) return rc;
-  if ((rc = fft_pass<R, false>(ctx, s, work, m, 1, n[0], m[2]))) return rc;
-  if ((rc = fft_pass<R, false>(ctx, s, work, m, 0, m[1], m[2]))) return rc;
-  const unsigned blocks = (unsigned)((M + 255) / 256);
+  // forward: z on lines (i < n0, j < n1), y on (i < n0, all k), x on all (j, k); inverse in reverse
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 2, n[0], n[1], n[2], m[2]))) return rc;
+  if ((rc = fft_pass<R, false>(ctx, s, work, m, 1, n[0], m[2], n[1], m[1]))) return rc;
   if (fs) {
-    hipLaunchKernelGGL(k_mul_real<R>, dim3(blocks), dim3(256), 0, s, work, gh, M);
+    // x: forward, times G^, inverse, keeping i < n0
+    if ((rc = fft_pass<R, false>(ctx, s, work, m, 0, m[1], m[2], n[0], n[0], &gv))) return rc;
   } else {
+    if ((rc = fft_pass<R, false>(ctx, s, work, m, 0, m[1], m[2], m[0], m[0]))) return rc;
+    const unsigned blocks = (unsigned)((M + 255) / 256);
     const double step_ang = g->step / kAng2Bor;
     hipLaunchKernelGGL(k_poisson_periodic<R>, dim3(blocks), dim3(256), 0, s, work, m[0], m[1], m[2], step_ang,
                        1.0 / (double)M);
+    QM_CHECK_LAUNCH(ctx);
+    if ((rc = fft_pass<R, true>(ctx, s, work, m, 0, m[1], m[2], m[0], m[0]))) return rc;
   }
-  QM_CHECK_LAUNCH(ctx);
-  // inverse: x on all (j, k), y on (i < n0, all k), z on (i < n0, j < n1)
-  if ((rc = fft_pass<R, true>(ctx, s, work, m, 0, m[1], m[2]))) return rc;
-  if ((rc = fft_pass<R, true>(ctx, s, work, m, 1, n[0], m[2]))) return rc;
-  if ((rc = fft_pass<R, true>(ctx, s, work, m, 2, n[0], n[1]))) return rc;
+  if ((rc = fft_pass<R, true>(ctx, s, work, m, 1, n[0], m[2], m[1], n[1]))) return rc;
+  if ((rc = fft_pass<R, true>(ctx, s, work, m, 2, n[0], n[1], m[2], n[2]))) return rc;
   const int64_t N = (int64_t)n[0] * n[1] * n[2];
   hipLaunchKernelGGL((k_extract<R, TIO>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, (const Cx<R>*)work,
                      m[1], m[2], n[0], n[1], n[2], out);

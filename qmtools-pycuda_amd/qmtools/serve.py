@@ -126,14 +126,32 @@ def freespace_len(n):
         v += 1
 
 
+def _padded_shape(req):
+    """Shape of the request's padded free-space FFT grid, or None without a grid shape."""
+    shape = getattr(getattr(req, "grid", None), "shape", None)
+    if shape is None or len(shape) < 3:
+        return None
+    return tuple(freespace_len(int(d)) for d in shape[-3:])
+
+
 def _padded_points(req):
     """Points of the request's padded free-space FFT grid (an upper bound of 27 x npts without a grid
     shape: each axis pads to less than 3 n)."""
-    shape = getattr(getattr(req, "grid", None), "shape", None)
-    if shape is None or len(shape) < 3:
-        return 27 * req.npts
-    nx, ny, nz = (int(d) for d in shape[-3:])
-    return freespace_len(nx) * freespace_len(ny) * freespace_len(nz)
+    m = _padded_shape(req)
+    return 27 * req.npts if m is None else m[0] * m[1] * m[2]
+
+
+def _green_points(reqs):
+    """Values of the cached Green's-function transforms the batch needs: one octant,
+    (m0/2 + 1)(m1/2 + 1)(m2/2 + 1), per distinct padded shape (at most 5 npts without a shape)."""
+    shapes, loose = set(), 0
+    for r in reqs:
+        m = _padded_shape(r)
+        if m is None:
+            loose += 5 * r.npts
+        else:
+            shapes.add(m)
+    return loose + sum((a // 2 + 1) * (b // 2 + 1) * (c // 2 + 1) for a, b, c in shapes)
 
 
 def workspace_estimate(reqs, precision="mixed", poisson="jacobi"):
@@ -146,7 +164,7 @@ def workspace_estimate(reqs, precision="mixed", poisson="jacobi"):
     if poisson == "fft":
         cplx = 16 if precision == "fp64" else 8
         padded = [_padded_points(r) for r in reqs]
-        hartree = max(padded, default=0) * cplx + min(_GREEN_BUDGET, sum(m * elt for m in padded))
+        hartree = max(padded, default=0) * cplx + min(_GREEN_BUDGET, _green_points(reqs) * elt)
     else:
         hartree = 0 if precision == "fp64" else 12 * npts
     return phi + hartree

@@ -346,14 +346,22 @@ def batched_section(basis, count, sweeps, seed=41):
     for poisson in ("jacobi", "fft"):
         batch.run(mols[:2], grids[:2], "mixed", 100, lane=lane, poisson=poisson)     # warm-up
         torch.cuda.synchronize()
+        # density, then Poisson, for the whole batch on each side (the split costs one synchronization)
         t0 = time.perf_counter()
-        qs, vs = batch.run(mols, grids, "mixed", sweeps, lane=lane, poisson=poisson)
+        qs = batch.density_batch(mols, grids, "mixed", lane)
         torch.cuda.synchronize()
-        t_mixed = time.perf_counter() - t0
-        t0 = time.perf_counter()
+        t1 = time.perf_counter()
+        if poisson == "jacobi":
+            vs = batch.hartree_jacobi_batch(mols, grids, qs, sweeps, "mixed", "multipole", lane)
+        else:
+            vs = batch.hartree_fft_batch(mols, grids, qs, "mixed", lane=lane)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        dens = [qmtools.QMTools.Compute_density(g, m, precision="fp64", copyBack=False) for m, g in zip(mols, grids)]
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
         plain = []
-        for m, g in zip(mols, grids):
-            q = qmtools.QMTools.Compute_density(g, m, precision="fp64", copyBack=False)
+        for m, g, q in zip(mols, grids, dens):
             if poisson == "jacobi":
                 v = qmtools.QMTools.Compute_hartree_iter(q, m, tolerance=None, max_sweeps=sweeps,
                                                          boundary="multipole", copyBack=False).d_qube
@@ -361,12 +369,17 @@ def batched_section(basis, count, sweeps, seed=41):
                 v = batch.hartree_fft_batch([m], [g], [q.d_qube], "fp64", lane=lane)[0]
             plain.append((q.d_qube, v))
         torch.cuda.synchronize()
-        t_plain = time.perf_counter() - t0
+        t4 = time.perf_counter()
+        del dens
+        t_mixed, t_plain = t2 - t0, t4 - t2
+        parts = {"density": {"mixed_batched": t1 - t0, "plain_fp64": t3 - t2},
+                 "poisson": {"mixed_batched": t2 - t1, "plain_fp64": t4 - t3}}
         rho_err = max(float((q.double() - p[0]).abs().max() / p[0].abs().max()) for q, p in zip(qs, plain))
         chg_err = max(float(abs(q.double().sum() - p[0].sum()) / p[0].sum()) for q, p in zip(qs, plain))
         v_err = max(float((v.double() - p[1]).abs().max() / p[1].abs().max()) for v, p in zip(vs, plain))
         out[poisson] = {"mixed_batched_pts_per_s": npts / t_mixed, "plain_fp64_pts_per_s": npts / t_plain,
                         "speedup": t_plain / t_mixed, "seconds": {"mixed_batched": t_mixed, "plain_fp64": t_plain},
+                        "seconds_by_part": parts,
                         "max_rel_err_vs_fp64": {"rho": rho_err, "charge": chg_err, "v_hartree": v_err}}
         del qs, vs, plain
     return out

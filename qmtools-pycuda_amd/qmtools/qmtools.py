@@ -114,6 +114,15 @@ def _as_device(grid, dtype):
     return t if t.dtype == want else t.to(want)
 
 
+def _result_grid(template, dtype):
+    """A new single-field grid with the geometry of `template` for a computed result: its device
+    buffer is allocated and cleared on the device, and the host `qube` keeps the zeros of a fresh
+    grid (nothing is copied between host and device)."""
+    g = Grid.emptyAs(template, nfields=1, dtype=dtype)
+    g.d_qube = _device.zeros(g.qube.size, dtype)
+    return g
+
+
 class QMTools:
     """Grid hot-path compute API (reference `qmtools.py:2320-2831`)."""
 
@@ -129,7 +138,7 @@ class QMTools:
         """
         p = _prec(precision)
         dt = numpy.float64 if p == "fp64" else numpy.float32
-        cgrid = Grid.emptyAs(grid, nfields=1, dtype=dt)
+        cgrid = _result_grid(grid, dt)
         dm = device_molecule(molecule)
         raw = ctypes.c_double(0.0)
         nat.call("qm_density", dm.handle, nat.stream(), ctypes.byref(nat.geom(cgrid)), int(subgrid), _PREC[p],
@@ -184,9 +193,8 @@ class QMTools:
         p = _prec(precision)
         seed = QMTools.Compute_qseed(qgrid, molecule, copyBack=False)
         rho = seed.d_qube[: int(qgrid.npts)] - _as_device(qgrid, numpy.float64)
-        vgrid = Grid.emptyAs(qgrid, nfields=1, dtype=numpy.float64)
+        vgrid = _result_grid(qgrid, numpy.float64)
         v = vgrid.d_qube
-        v.zero_()
         g = nat.geom(vgrid)
         if boundary == "multipole":
             mom = (ctypes.c_double * 10)()
@@ -241,8 +249,7 @@ class QMTools:
     @staticmethod
     def Compute_qseed(grid, molecule, copyBack=True):
         """Nuclear charges spread on the 8 surrounding nodes (reference `qmtools.py:2645-2672`)."""
-        qgrid = Grid.emptyAs(grid, nfields=1, dtype=numpy.float64)
-        qgrid.d_qube.zero_()
+        qgrid = _result_grid(grid, numpy.float64)
         dm = device_molecule(molecule)
         nat.call("qm_seed", dm.handle, nat.stream(), ctypes.byref(nat.geom(qgrid)), nat.QM_F64,
                  nat.dptr(qgrid.d_qube))
@@ -264,8 +271,8 @@ class QMTools:
         """
         p = _prec(precision)
         dt = numpy.float64 if p == "fp64" else numpy.float32
-        gres = Grid.emptyAs(grid, nfields=1, dtype=dt)
-        tmp = Grid.emptyAs(grid, nfields=1, dtype=dt)
+        gres = _result_grid(grid, dt)
+        tmp = _result_grid(grid, dt)
         if maxiters is None:
             maxiters = 10 * numpy.power(grid.npts, 1.0 / 3)
         dm = device_molecule(molecule)
@@ -288,7 +295,7 @@ class QMTools:
     @staticmethod
     def Compute_VNe_classic(grid, molecule, divisions=4, copyBack=True):
         """Classical nuclear potential averaged over divisions^3 sub-points (reference `qmtools.py:2750-2790`)."""
-        gout = Grid.emptyAs(grid, nfields=1, dtype=numpy.float64)
+        gout = _result_grid(grid, numpy.float64)
         dm = device_molecule(molecule)
         nat.call("qm_vne_classic", dm.handle, nat.stream(), ctypes.byref(nat.geom(gout)), int(divisions),
                  nat.QM_F64, nat.dptr(gout.d_qube))

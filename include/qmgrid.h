@@ -288,7 +288,9 @@ enum {
                                diagonal skipping (default) */
   QM_TUNE_JACOBI_ZS = 2,    /* z planes per Jacobi sweep workgroup, 0 = chosen per grid (the two-sweep passes
                                of qm_jacobi_batch: one length for the whole batch), else 1..4096 */
-  QM_TUNE_COUNT = 3
+  QM_TUNE_JACOBI_PAIRS = 3, /* FP32 two-sweep Jacobi passes: 1 = two columns per lane on grids with an even nx
+                               (default), 0 = one column per lane; the results are bitwise the same */
+  QM_TUNE_COUNT = 4
 };
 int qm_tune_set(qm_ctx* ctx, int knob, int value);
 int qm_tune_get(qm_ctx* ctx, int knob, int* value);

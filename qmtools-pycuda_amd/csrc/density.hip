@@ -2575,6 +2575,11 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
     ctx->jacobi_zs = value;
     return QM_OK;
   }
+  if (knob == QM_TUNE_JACOBI_PAIRS) {
+    QM_REQUIRE(ctx, value == 0 || value == 1, QM_EINVAL, "tune: Jacobi pairs must be 0 or 1");
+    ctx->jacobi_pairs = value;
+    return QM_OK;
+  }
   return qm_fail(ctx, QM_EINVAL, "tune: unknown knob %d", knob);
 }
 
@@ -2592,6 +2597,10 @@ int qm_tune_get(qm_ctx* ctx, int knob, int* value) {
   }
   if (knob == QM_TUNE_JACOBI_ZS) {
     *value = ctx->jacobi_zs;
+    return QM_OK;
+  }
+  if (knob == QM_TUNE_JACOBI_PAIRS) {
+    *value = ctx->jacobi_pairs;
     return QM_OK;
   }
   return qm_fail(ctx, QM_EINVAL, "tune: unknown knob %d", knob);

@@ -304,6 +304,158 @@ inline int pick_zs2(int nx, int ny, int nz, int fixed) {
   return std::max(4, (nz + tz - 1) / tz);
 }
 
+// FP32 two-sweep pass with two adjacent columns per lane (8-byte loads and stores, half the
+// lane shuffles per point).  A strip spans 128 columns, lane L holding columns i and i + 1 with
+// i = 124 bx - 2 + 2L; the first sweep is formed on all 128 columns and the second on columns
+// 2 ... 125, the strip's 124 outputs.  The first sweep at the two outer columns sees a clamped
+// or the lane's own value as its missing neighbour; it feeds only the second sweep at columns 1
+// and 126, which is not stored, so no halo column is loaded.  nx must be even: pairs start at
+// even columns and lie wholly inside or outside the grid.  Each point's arithmetic and summation
+// order are those of the one-column kernels, so the results are bitwise the same.
+constexpr int kWP = 2 * kSW - 4;   // output columns per strip
+constexpr int kR2P = 4;            // rows per strip
+template <int R>
+struct Strip2P {
+  static constexpr int W = kWP;
+  static constexpr int TY = R * kSWaves;
+};
+
+template <int R>
+__device__ __forceinline__ void strip2p_body(int bx, int by, int bz, int zs, const float* __restrict__ rho,
+                                             const float* __restrict__ v, float* __restrict__ vout, int nx, int ny,
+                                             int nz, float h) {
+  constexpr int R0 = R + 4, R1 = R + 2;
+  const int lane = threadIdx.x;
+  const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int i = bx * kWP - 2 + 2 * lane;   // first column of the lane's pair (even)
+  const int j0 = by * Strip2P<R>::TY + wy * R;
+  if (j0 >= ny) return;                    // whole wave idle; the kernel has no barrier
+  const int z0 = bz * zs, z1 = min(nz, z0 + zs);
+  const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+  // clamped pair: values read at it only feed outer-layer points (copied) or points outside the grid
+  const unsigned ic = (unsigned)min(max(i, 0), nx - 2);
+  auto row = [&](int j) -> size_t { return sy * (size_t)min(max(j, 0), ny - 1); };
+  auto pln = [&](int k) -> size_t { return sz * (size_t)min(max(k, 0), nz - 1); };
+  auto ld = [](const float* __restrict__ p, size_t off) -> float2 { return *(const float2*)(p + off); };
+  const float hinv = 1.0f / h;
+  const bool xb0 = i <= 0 || i >= nx - 1, xb1 = i + 1 <= 0 || i + 1 >= nx - 1;
+  const bool store = lane >= 1 && lane <= kSW - 2 && i < nx;
+  float2 am[R0], ac[R0], ap[R0];   // V on planes m-1, m, m+1, rows j0-2+q
+  float2 bmm[R1], bm[R1];          // first sweep on planes m-2, m-1, rows j0-1+p
+  float2 rc[R1], rm[R1];           // rho of planes m and m-1 (rows j0-1+p)
+  // loads two planes ahead into ping-pong buffers, as in the one-column kernel
+  float2 an[2][R0], rn[2][R1];
+  const int m0 = z0 - 1;
+  auto request = [&](int m, auto buf) __attribute__((always_inline)) {
+    constexpr int b = decltype(buf)::value;
+    const size_t pv = pln(m + 3), pr = pln(m + 2);
+#pragma unroll
+    for (int q = 0; q < R0; ++q) an[b][q] = ld(v, pv + row(j0 - 2 + q) + ic);
+#pragma unroll
+    for (int p = 0; p < R1; ++p) rn[b][p] = ld(rho, pr + row(j0 - 1 + p) + ic);
+  };
+#pragma unroll
+  for (int q = 0; q < R0; ++q) {
+    am[q] = ld(v, pln(m0 - 1) + row(j0 - 2 + q) + ic);
+    ac[q] = ld(v, pln(m0) + row(j0 - 2 + q) + ic);
+    ap[q] = ld(v, pln(m0 + 1) + row(j0 - 2 + q) + ic);
+    an[1][q] = ld(v, pln(m0 + 2) + row(j0 - 2 + q) + ic);
+  }
+#pragma unroll
+  for (int p = 0; p < R1; ++p) {
+    rc[p] = ld(rho, pln(m0) + row(j0 - 1 + p) + ic);
+    rn[1][p] = ld(rho, pln(m0 + 1) + row(j0 - 1 + p) + ic);
+    rm[p] = make_float2(0.f, 0.f);
+    bmm[p] = make_float2(0.f, 0.f);
+    bm[p] = make_float2(0.f, 0.f);
+  }
+  // V' at the pair from its x neighbours (l, c.x) and (c.y, r) and the four others
+  auto relax = [&](float2 c, float l, float r, float2 yp, float2 ym, float2 zp, float2 zm, float2 q, bool o0,
+                   bool o1) __attribute__((always_inline)) -> float2 {
+    float s0 = l + c.y;
+    s0 += yp.x;
+    s0 += ym.x;
+    s0 += zp.x;
+    s0 += zm.x;
+    s0 = fmaf(-q.x, hinv, s0);
+    float s1 = c.x + r;
+    s1 += yp.y;
+    s1 += ym.y;
+    s1 += zp.y;
+    s1 += zm.y;
+    s1 = fmaf(-q.y, hinv, s1);
+    return make_float2(o0 ? c.x : s0 * (float)(1.0 / 6.0), o1 ? c.y : s1 * (float)(1.0 / 6.0));
+  };
+  auto step = [&](int m, auto buf) __attribute__((always_inline)) {
+    constexpr int b = decltype(buf)::value;
+    request(m, buf);
+    // first sweep on plane m
+    const bool zb1 = m <= 0 || m >= nz - 1;
+    float2 bc[R1];
+#pragma unroll
+    for (int p = 0; p < R1; ++p) {
+      const int j = j0 - 1 + p;
+      const float2 c = ac[p + 1];
+      const float l = __shfl_up(c.y, 1, kSW);
+      const float r = __shfl_down(c.x, 1, kSW);
+      const bool o = zb1 || j <= 0 || j >= ny - 1;
+      bc[p] = relax(c, l, r, ac[p + 2], ac[p], ap[p + 1], am[p + 1], rc[p], o || xb0, o || xb1);
+    }
+    // second sweep on plane m-1
+    const int k = m - 1;
+    if (k >= z0) {
+      const bool zb2 = k == 0 || k == nz - 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        const float2 c = bm[r + 1];
+        const float l = __shfl_up(c.y, 1, kSW);
+        const float rr = __shfl_down(c.x, 1, kSW);
+        const bool o = zb2 || j == 0 || j == ny - 1;
+        const float2 val = relax(c, l, rr, bm[r + 2], bm[r], bc[r + 1], bmm[r + 1], rm[r + 1], o || xb0, o || xb1);
+        if (store && j < ny) *(float2*)(vout + sz * (size_t)k + sy * (size_t)j + (unsigned)i) = val;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      am[q] = ac[q];
+      ac[q] = ap[q];
+      ap[q] = an[1 - b][q];
+    }
+#pragma unroll
+    for (int p = 0; p < R1; ++p) {
+      bmm[p] = bm[p];
+      bm[p] = bc[p];
+      rm[p] = rc[p];
+      rc[p] = rn[1 - b][p];
+    }
+  };
+  int m = m0;
+  for (; m + 1 <= z1; m += 2) {
+    step(m, std::integral_constant<int, 0>());
+    step(m + 1, std::integral_constant<int, 1>());
+  }
+  if (m <= z1) step(m, std::integral_constant<int, 0>());
+}
+
+template <int R>
+__global__ __launch_bounds__(kSW* kSWaves) void k_sweep_strip2p(const float* __restrict__ rho,
+                                                                const float* __restrict__ v, float* __restrict__ vout,
+                                                                int nx, int ny, int nz, int zs, float h) {
+  strip2p_body<R>(blockIdx.x, blockIdx.y, blockIdx.z, zs, rho, v, vout, nx, ny, nz, h);
+}
+
+// planes per workgroup of the paired two-sweep kernel (as pick_zs2)
+// the paired kernel's grids: even nx
+inline bool pairs_fit(int nx, int ny) { return nx % 2 == 0 && ny > 0; }
+
+inline int pick_zs2p(int nx, int ny, int nz, int fixed) {
+  if (fixed > 0) return std::min(fixed, nz);
+  const int tiles = ((nx + kWP - 1) / kWP) * ((ny + Strip2P<kR2P>::TY - 1) / Strip2P<kR2P>::TY);
+  const int tz = std::max(1, 1024 / tiles);
+  return std::max(4, (nz + tz - 1) / tz);
+}
+
 // planes per workgroup: about 1024 workgroups (4096 waves) for the chip, at least 4 planes each;
 // a positive `fixed` (the tuning knob) overrides
 inline int pick_zs(int nx, int ny, int nz, int fixed) {
@@ -457,6 +609,15 @@ int sweep(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const T* rho, const T* a
 template <typename T>
 int sweep2(qm_ctx* ctx, hipStream_t s, const qm_geom* g, const T* rho, const T* a, T* b, bool prof = true) {
   ProfScope ps(ctx, s, QM_K_JACOBI, 2, prof);
+  if (sizeof(T) == 4 && ctx->jacobi_pairs && pairs_fit(g->nx, g->ny)) {
+    const int zs = pick_zs2p(g->nx, g->ny, g->nz, ctx->jacobi_zs);
+    const dim3 grid((g->nx + kWP - 1) / kWP, (g->ny + Strip2P<kR2P>::TY - 1) / Strip2P<kR2P>::TY,
+                    (g->nz + zs - 1) / zs);
+    hipLaunchKernelGGL(k_sweep_strip2p<kR2P>, grid, dim3(kSW, kSWaves), 0, s, (const float*)rho, (const float*)a,
+                       (float*)b, g->nx, g->ny, g->nz, zs, (float)g->step);
+    QM_CHECK_LAUNCH(ctx);
+    return QM_OK;
+  }
   constexpr int R = r2<T>();
   const int zs = pick_zs2<T>(g->nx, g->ny, g->nz, ctx->jacobi_zs);
   const dim3 grid((g->nx + Strip2<R>::W - 1) / Strip2<R>::W, (g->ny + Strip2<R>::TY - 1) / Strip2<R>::TY,
@@ -779,7 +940,7 @@ struct JItem {
   float* r32;
   int blk0, bx, by;          // pointwise kernels: 64 x 4 tiles, one z plane per block
   int blk0s, tbx, tby, zs;   // sweep: 64 x 32 tiles (four 64 x 8 strips), zs planes per block
-  int blk0t, tbx2, tby2, zs2;  // two-sweep pass: 62-column strips
+  int blk0t, tbx2, tby2, zs2;  // two-sweep pass: 62-column strips (124 with paired columns)
 };
 
 __device__ __forceinline__ int find_item(const JItem* __restrict__ it, int n, int b) {
@@ -832,6 +993,18 @@ __global__ __launch_bounds__(kSW* kSWaves) void kb_sweep2(const JItem* __restric
   const int bz = rest / q.tby2;
   strip2_body<T, r2<T>()>(bx, by, bz, q.zs2, (const T*)q.rho, (const T*)(dir ? q.b : q.a), (T*)(dir ? q.a : q.b),
                           q.nx, q.ny, q.nz, (T)q.h);
+}
+
+template <int R>
+__global__ __launch_bounds__(kSW* kSWaves) void kb_sweep2p(const JItem* __restrict__ items, int n, int dir) {
+  const JItem& q = items[find_item_t(items, n, blockIdx.x)];
+  const int local = blockIdx.x - q.blk0t;
+  const int bx = local % q.tbx2;
+  const int rest = local / q.tbx2;
+  const int by = rest % q.tby2;
+  const int bz = rest / q.tby2;
+  strip2p_body<R>(bx, by, bz, q.zs2, (const float*)q.rho, (const float*)(dir ? q.b : q.a), (float*)(dir ? q.a : q.b),
+                  q.nx, q.ny, q.nz, (float)q.h);
 }
 
 template <typename T>
@@ -946,7 +1119,11 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
   int blocks = 0, sblocks = 0, tblocks = 0;
   size_t f32_elems = 0;
   constexpr int RD = kR2D, RF = kR2F;
-  const int W = Strip2<RD>::W, TY = f64 ? Strip2<RD>::TY : Strip2<RF>::TY;
+  // FP32 two-sweep passes take the paired-column kernel when every grid has an even nx
+  bool pairs = !f64 && ctx->jacobi_pairs;
+  for (int i = 0; i < n && pairs; ++i) pairs = pairs_fit(geoms[i].nx, geoms[i].ny);
+  const int W = pairs ? kWP : Strip2<RD>::W;
+  const int TY = f64 ? Strip2<RD>::TY : pairs ? Strip2P<kR2P>::TY : Strip2<RF>::TY;
   // two-sweep passes: one slab length for the whole batch, for about 1024 workgroups in all
   // (per-grid slabs of small grids would be a few planes, each re-reading 4 halo planes)
   int64_t tile_planes = 0;
@@ -1027,8 +1204,10 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
       // the residual kernel forms the first sweep of the correction (into b); kf - 1 follow
       hipLaunchKernelGGL(kb_residual, grid, thr, 0, t, dt, n);
       dir = 1;
-      for (int64_t q = 0; q < (kf - 1) / 2; ++q, dir ^= 1)
-        hipLaunchKernelGGL(kb_sweep2<float>, tgrid, sthr, 0, t, dt, n, dir);
+      for (int64_t q = 0; q < (kf - 1) / 2; ++q, dir ^= 1) {
+        if (pairs) hipLaunchKernelGGL(kb_sweep2p<kR2P>, tgrid, sthr, 0, t, dt, n, dir);
+        else hipLaunchKernelGGL(kb_sweep2<float>, tgrid, sthr, 0, t, dt, n, dir);
+      }
       if ((kf - 1) & 1) {
         hipLaunchKernelGGL(kb_sweep<float>, sgrid, sthr, 0, t, dt, n, dir);
         dir ^= 1;
@@ -1048,9 +1227,9 @@ extern "C" int qm_jacobi_batch(qm_ctx* ctx, void* stream, int n, const qm_geom* 
     QM_CHECK_HIP(ctx, hipStreamWaitEvent(gs, ev, 0));
     // every kernel argument and grid of the block follows from the item table, its address, the
     // block length and the precision
-    std::vector<char> key(sizeof(int) * 3 + sizeof(void*) + tbytes);
+    std::vector<char> key(sizeof(int) * 4 + sizeof(void*) + tbytes);
     {
-      const int hdr[3] = {n, block, f64 ? 1 : 0};
+      const int hdr[4] = {n, block, f64 ? 1 : 0, pairs ? 1 : 0};
       std::memcpy(key.data(), hdr, sizeof(hdr));
       std::memcpy(key.data() + sizeof(hdr), &dtab, sizeof(void*));
       std::memcpy(key.data() + sizeof(hdr) + sizeof(void*), items.data(), tbytes);

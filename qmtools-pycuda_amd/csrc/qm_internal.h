@@ -112,6 +112,7 @@ struct qm_ctx {
   int split_tm = 3;                     // split contraction variant (0..3 tiled, 4 fused, 64 / 128 all-columns)
   int f64_variant = 4;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled, 3 rows, 4 tiled + DMA
   int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)
+  int jacobi_pairs = 1;                 // FP32 two-sweep passes with two columns per lane (even nx)
   int ncu = 256;                        // compute units of the device
   std::vector<ProfRec> recs;
 };

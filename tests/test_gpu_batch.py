@@ -184,3 +184,43 @@ def test_jacobi_rounds_alternating_sets_equal_one_call(items, precision):
         for st, w in zip(states[k], want):
             numpy.testing.assert_array_equal(st[1].cpu().numpy(), w.cpu().numpy())
     lane.close()
+
+
+def test_jacobi_fp32_paired_columns_bitwise(items):
+    """FP32 two-sweep passes with two columns per lane equal the one-column kernel bit for bit: single
+    grids (one strip, and 136 x 24 x 16 with two strips and a part-filled last one) and a batch, over
+    several graph replays and an odd remainder of sweeps."""
+    import qmtools
+    from qmtools import autotune, batch, synth
+    from qmtools.grid import Grid, make_float3
+    m0, g0 = items[0]
+    wide = Grid((1, 136, 24, 16), 1.0)
+    wide.step = numpy.float64(0.35)
+    c = m0.coords.mean(axis=0)
+    wide.origin = make_float3(*(c - 0.5 * 0.35 * numpy.array([136, 24, 16])))
+    cases = [(m0, g0), (m0, wide)]
+    sweeps = 3 * 50 + 7
+    old = autotune.get("jacobi_pairs")
+    out = {}
+    try:
+        for pairs in (1, 0):
+            autotune.set("jacobi_pairs", pairs)
+            res = []
+            for m, g in cases:
+                _say("jacobi fp32 pairs", pairs, g.shape, sweeps)
+                q = qmtools.QMTools.Compute_density(g, m, precision="mixed", copyBack=False)
+                res.append(qmtools.QMTools.Compute_hartree_iter(q, m, tolerance=None, max_sweeps=sweeps,
+                                                                boundary="multipole", precision="mixed").qube.ravel())
+            mols = [m for m, _ in items] + [m0]
+            grids = [g for _, g in items] + [wide]
+            _say("jacobi fp32 batch pairs", pairs, [g.shape for g in grids])
+            qs = batch.density_batch(mols, grids, "mixed")
+            res += [v.cpu().numpy() for v in batch.hartree_jacobi_batch(mols, grids, qs, sweeps, "mixed",
+                                                                          "multipole")]
+            out[pairs] = res
+    finally:
+        autotune.set("jacobi_pairs", old)
+    assert len(out[1]) == len(out[0]) == 2 + len(items) + 1
+    for a, b in zip(out[1], out[0]):
+        assert numpy.abs(b).max() > 0
+        numpy.testing.assert_array_equal(a, b)

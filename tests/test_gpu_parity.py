@@ -368,6 +368,36 @@ def test_fft_mixed_radix_lengths(qm, small, shape):
             assert _rel(v, want) <= tol, (mode, precision)
 
 
+def test_fft_freespace_green_cache_across_steps(qm, small):
+    """Grids of one shape and different steps share the cached Green's-function transform (kept at unit
+    step, scaled by h^2 at use): each step's free-space potential equals the oracle's (FP64 1e-10, FP32
+    1e-6), and the context's resident bytes do not grow after the first step."""
+    import ctypes
+    from oracle import fft as off
+    from qmtools import _native as nat
+    from qmtools.grid import make_float3
+    mol, _, _ = small
+    shape = (40, 32, 24)
+    c = mol.coords.mean(axis=0)
+    rng = numpy.random.default_rng(11)
+    dens = rng.uniform(0.0, 0.01, (1,) + shape)
+    resident = []
+    for step in (0.25, 0.31, 0.4):
+        grid = qm.Grid((1,) + shape, step)
+        grid.origin = make_float3(*(c - 0.5 * numpy.array(shape) * float(grid.step)))
+        grid.LoadFromNPY(dens)
+        nx, ny, nz, h, o = grid.geometry()
+        want = off.hartree_fft(grid.qube.ravel(), mol.types, mol.coords, nx, ny, nz, h, o, mode="freespace")
+        for precision, tol in (("fp64", 1e-10), ("mixed", 1e-6)):
+            _say("fft green cache", shape, step, precision)
+            v = qm.QMTools.Compute_hartree_fft(grid, mol, mode="freespace", precision=precision)
+            assert _rel(v, want) <= tol, (step, precision)
+        b = ctypes.c_int64(0)
+        nat.check(nat.library().qm_ctx_memory(nat.context(), ctypes.byref(b)), nat.context())
+        resident.append(int(b.value))
+    assert resident[1] == resident[0] and resident[2] == resident[0], resident
+
+
 def test_fft_fp32(qm, small):
     from oracle import fft as off
     mol, grid, _ = small

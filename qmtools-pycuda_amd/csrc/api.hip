@@ -109,6 +109,7 @@ int qm_ctx_destroy(qm_ctx* ctx) {
   if (ctx->jgraph) (void)hipGraphDestroy(ctx->jgraph);
   if (ctx->table_uploaded) (void)hipEventDestroy(ctx->table_uploaded);
   if (ctx->table_pinned) (void)hipHostFree(ctx->table_pinned);
+  if (ctx->gto_staged) (void)hipHostFree(ctx->gto_staged);
   qm_fft_cache_free(ctx->fft);
   clear_records(ctx);
   if (ctx->graph_stream) (void)hipStreamDestroy(ctx->graph_stream);

@@ -24,7 +24,10 @@
 // downward recursion F_{n-1} = (2T F_n + e^-T)/(2n-1); for T >= 40,
 // F_0 = sqrt(pi/T)/2 and upward recursion (e^-40 is below FP64 resolution of F).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "qm_internal.h"
@@ -293,8 +296,25 @@ void hermite_E(double a, double b, double XAB, double E[3][3][6]) {
     }
 }
 
+// f(i) for every i in [0, n), taken in increasing i by up to 16 host threads (the calling thread
+// included; with fewer threads available it takes what the others leave)
+template <typename F>
+void host_parallel(int n, F&& f) {
+  std::atomic<int> next{0};
+  auto runner = [&]() {
+    for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
+  };
+  const int nthr = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  try {
+    for (int t = 1; t < nthr && t < n; ++t) pool.emplace_back(runner);
+  } catch (const std::system_error&) {
+  }
+  runner();
+  for (auto& t : pool) t.join();
+}
+
 struct GtoCache {
-  std::vector<HDist> lists[kLmax + 1];
   HDist* dev = nullptr;
   int off[kLmax + 1] = {0, 0, 0, 0, 0};
   int n[kLmax + 1] = {0, 0, 0, 0, 0};
@@ -307,8 +327,14 @@ int build_distributions(qm_ctx* ctx, qm_mol* mol, hipStream_t s, GtoCache& c) {
   std::vector<CartTerm> terms[3][5];
   for (int L = 0; L <= 2; ++L)
     for (int mi = 0; mi < 2 * L + 1; ++mi) terms[L][mi] = cart_terms(L, mi);
-  for (auto& l : c.lists) l.clear();
-  for (int sa = 0; sa < nsh; ++sa) {
+  // distributions of shell A with the shells B <= A, per L, in the order (B, primitive of A,
+  // primitive of B); the table is the concatenation over A in increasing order, the same whether
+  // the shells are built by one thread or several
+  struct ShellOut {
+    std::vector<HDist> l[kLmax + 1];
+  };
+  std::vector<ShellOut> per(nsh);
+  auto build_shell = [&](int sa) {
     const DevShell& A = mol->h_shells[sa];
     const double* RA = &mol->xyz[3 * A.atom];
     for (int sb = 0; sb <= sa; ++sb) {
@@ -389,16 +415,23 @@ int build_distributions(qm_ctx* ctx, qm_mol* mol, hipStream_t s, GtoCache& c) {
                 bound += std::fabs(h.D[tuv_index(t, u, v)]) * g;
               }
           if (bound < kGtoScreen) continue;
-          c.lists[L].push_back(h);
+          per[sa].l[L].push_back(h);
         }
       }
     }
-  }
+  };
+  // shells taken largest first (shell A pairs with A + 1 shells)
+  host_parallel(nsh, [&](int k) { build_shell(nsh - 1 - k); });
+  // table offsets: L-major, shells in increasing order within each L
+  std::vector<size_t> at((size_t)(kLmax + 1) * nsh);
   size_t tot = 0;
   for (int L = 0; L <= kLmax; ++L) {
     c.off[L] = (int)tot;
-    c.n[L] = (int)c.lists[L].size();
-    tot += c.lists[L].size();
+    for (int sa = 0; sa < nsh; ++sa) {
+      at[(size_t)L * nsh + sa] = tot;
+      tot += per[sa].l[L].size();
+    }
+    c.n[L] = (int)(tot - c.off[L]);
   }
   // stream-ordered: kernels queued earlier on s still read the previous table
   if (c.dev) {
@@ -406,12 +439,25 @@ int build_distributions(qm_ctx* ctx, qm_mol* mol, hipStream_t s, GtoCache& c) {
     c.dev = nullptr;
   }
   if (tot) {
+    // the context's pinned staging buffer (the last upload from it finished: synchronised below)
+    const size_t bytes = sizeof(HDist) * tot;
+    if (bytes > ctx->gto_staged_bytes) {
+      if (ctx->gto_staged) QM_CHECK_HIP(ctx, hipHostFree(ctx->gto_staged));
+      ctx->gto_staged = nullptr;
+      ctx->gto_staged_bytes = 0;
+      QM_CHECK_HIP(ctx, hipHostMalloc(&ctx->gto_staged, bytes, hipHostMallocDefault));
+      ctx->gto_staged_bytes = bytes;
+    }
+    HDist* staged = (HDist*)ctx->gto_staged;
+    // the shells' lists go to their places in the staging buffer
+    host_parallel(nsh, [&](int sa) {
+      for (int L = 0; L <= kLmax; ++L)
+        if (!per[sa].l[L].empty())
+          std::copy(per[sa].l[L].begin(), per[sa].l[L].end(), staged + at[(size_t)L * nsh + sa]);
+    });
     QM_CHECK_HIP(ctx, hipMallocAsync((void**)&c.dev, sizeof(HDist) * tot, s));
-    for (int L = 0; L <= kLmax; ++L)
-      if (c.n[L])
-        QM_CHECK_HIP(ctx, hipMemcpyAsync(c.dev + c.off[L], c.lists[L].data(), sizeof(HDist) * c.n[L],
-                                         hipMemcpyHostToDevice, s));
-    QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // the host lists are rebuilt in place next time
+    QM_CHECK_HIP(ctx, hipMemcpyAsync(c.dev, staged, bytes, hipMemcpyHostToDevice, s));
+    QM_CHECK_HIP(ctx, hipStreamSynchronize(s));   // the staging buffer is rewritten by the next build
   }
   mol->n_gto_builds++;
   if (!c.boys) {

@@ -109,6 +109,8 @@ struct qm_ctx {
   void* table_pinned = nullptr;
   size_t table_pinned_bytes = 0;
   hipEvent_t table_uploaded = nullptr;  // recorded after the last upload from table_pinned
+  void* gto_staged = nullptr;           // pinned staging of GTO distribution tables (grown as needed;
+  size_t gto_staged_bytes = 0;          // every upload from it completes before the build returns)
   int split_tm = 3;                     // split contraction variant (0..3 tiled, 4 fused, 64 / 128 all-columns)
   int f64_variant = 4;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled, 3 rows, 4 tiled + DMA
   int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)

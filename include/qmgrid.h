@@ -290,7 +290,10 @@ enum {
                                of qm_jacobi_batch: one length for the whole batch), else 1..4096 */
   QM_TUNE_JACOBI_PAIRS = 3, /* FP32 two-sweep Jacobi passes: 1 = two columns per lane on grids with an even nx
                                (default), 0 = one column per lane; the results are bitwise the same */
-  QM_TUNE_COUNT = 4
+  QM_TUNE_TILE_SKIP = 4,    /* FP64 contraction with LDS DMA: 1 = skip the products of basis-function groups
+                               that are zero on a whole row tile (default), 0 = form every product; they add
+                               exact zeros, so the results are the same */
+  QM_TUNE_COUNT = 5
 };
 int qm_tune_set(qm_ctx* ctx, int knob, int value);
 int qm_tune_get(qm_ctx* ctx, int knob, int* value);

@@ -762,7 +762,8 @@ __device__ __forceinline__ void tiled_f64_body(int rt, int cb, const double* __r
 template <int KS, int NB>
 __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double* __restrict__ phi, int ld,
                                                    const double* __restrict__ M, int np, int nbf, int ncb,
-                                                   double* __restrict__ part, int ldp) {
+                                                   double* __restrict__ part, int ldp,
+                                                   const uint32_t* __restrict__ amask = nullptr, int nw = 0) {
   static_assert(KS % 4 == 0 && (2 * KS) % 8 == 0, "stages of whole MFMA k-steps, rows split over 8 waves");
   static_assert(KS == 16, "the diagonal stages below are the block's 8 tiles of 16 columns");
   constexpr int RPW = 2 * KS / 8;   // rows of one operand fetched per wave and stage
@@ -775,37 +776,81 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
   // padding, as columns below 0; rows and columns of M from nbf on are zero and are not visited.
   const int ne = (nbf + 15) / 16 * 16;
   const int row0 = rt * kTT, c0 = cb * kTT - (ncb * kTT - ne);
-  const int nst = (c0 + kTT) / KS;
   const int jd = c0 / KS;   // c0 is a multiple of 16: stage jd + j is the block's diagonal stage j
   // wave w brings k rows RPW (w % 4) .. RPW (w % 4) + RPW - 1 of operand w / 4 (0: basis values, 1: M)
   const int op = wave >> 2, k0 = RPW * (wave & 3);
   const double* src = op == 0 ? phi + row0 + 2 * lane : M + min(max(c0 + 2 * lane, 0), np - 2);   // clamped: unused
   const int64_t lds_ld = op == 0 ? ld : np;
-  auto issue = [&](int st) __attribute__((always_inline)) {
+
+  // Wave w owns rows 16 w .. 16 w + 15 and all 8 tiles of 16 columns of the block, so every wave
+  // drops the same tiles out of the triangle and completes the same tile in each diagonal stage.
+  // Groups of 16 functions whose basis values are all zero on this row tile (activity bit clear,
+  // k_tile_mask) add only zero products.  A block with no active group among its own columns writes
+  // zeros before its first load; otherwise the stages below the block's diagonal run only for the
+  // active groups (their list is compacted into LDS), loads included, and the eight diagonal stages
+  // always run.  Every decision is uniform per workgroup.
+  const uint32_t* __restrict__ mk = amask ? amask + (int64_t)rt * nw : nullptr;
+  auto act = [&](int st) __attribute__((always_inline)) -> bool {
+    return !mk || ((mk[st >> 5] >> (st & 31)) & 1u);
+  };
+  if (mk) {
+    bool any = false;
+    for (int j = 0; j < 8; ++j) any = any || (jd + j >= 0 && act(jd + j));
+    if (!any) {
+      if (lk == 0) part[(int64_t)cb * ldp + row0 + 16 * wave + li] = 0.0;
+      return;
+    }
+  }
+  constexpr int kSeqMax = 256;   // below-diagonal stages listed in LDS (16 kSeqMax functions)
+  __shared__ int seq[kSeqMax + 1];
+  const bool compact = mk && jd <= kSeqMax;
+  int nloop = jd > 0 ? jd : 0;
+  if (compact) {
+    if (wave == 0) {   // wave 0 lists the active stages below the diagonal in order
+      int cnt = 0;
+      for (int b = 0; b < jd; b += 64) {
+        const int st = b + lane;
+        const bool a = st < jd && act(st);
+        const uint64_t bal = __ballot(a);
+        if (a) seq[cnt + __popcll(bal & ((1ull << lane) - 1ull))] = st;
+        cnt += __popcll(bal);
+      }
+      if (lane == 0) seq[kSeqMax] = cnt;
+    }
+    __syncthreads();
+    nloop = __builtin_amdgcn_readfirstlane(seq[kSeqMax]);
+  }
+  // the ring's stage sequence: position p < nloop is a listed stage below the diagonal, then the
+  // diagonal stages jd + j0 .. jd + 7 (j0 > 0 only for block 0, whose first columns are padding)
+  const int j0 = jd < 0 ? -jd : 0;
+  const int npos = nloop + 8 - j0;
+  auto stage_of = [&](int q) __attribute__((always_inline)) -> int {
+    return q < nloop ? (compact ? __builtin_amdgcn_readfirstlane(seq[q]) : q) : jd + j0 + (q - nloop);
+  };
+  auto issue = [&](int q) __attribute__((always_inline)) {
+    const int st = stage_of(q);
 #pragma unroll
     for (int j = 0; j < RPW; ++j) {
       const int k = k0 + j;
-      lds_dma16(src + (int64_t)(st * KS + k) * lds_ld, &ring[st % NB][op][k][0]);
+      lds_dma16(src + (int64_t)(st * KS + k) * lds_ld, &ring[q % NB][op][k][0]);
     }
   };
-  // Wave w owns rows 16 w .. 16 w + 15 and all 8 tiles of 16 columns of the block, so every wave
-  // drops the same tiles out of the triangle and completes the same tile in each diagonal stage.
   d4 acc[8];
   double pr = 0.0;   // this lane's row-dot partial sum (row 16 w + li)
 #pragma unroll
   for (int c = 0; c < 8; ++c) acc[c] = d4{0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < NB - 1; ++q)
-    if (q < nst) issue(q);
-  // opens stage st: this thread's loads of st are done once at most RPW (NB - 2) newer ones are
-  // pending, everyone's after the barrier, and everyone is done with st - 1, whose slot is refilled
-  auto open = [&](int st) __attribute__((always_inline)) {
-    const int newer = min(NB - 2, nst - 1 - st);
+    if (q < npos) issue(q);
+  // opens position q: this thread's loads of q are done once at most RPW (NB - 2) newer ones are
+  // pending, everyone's after the barrier, and everyone is done with q - 1, whose slot is refilled
+  auto open = [&](int q) __attribute__((always_inline)) {
+    const int newer = min(NB - 2, npos - 1 - q);
     if (RPW == 2 && newer >= 2) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
     else if (RPW == 2 && newer == 1) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
     else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    if (st + NB - 1 < nst) issue(st + NB - 1);
+    if (q + NB - 1 < npos) issue(q + NB - 1);
   };
   // MFMAs of one stage for the tiles first .. 7 (transposed tiles: M first)
   auto body = [&](int buf, auto first) __attribute__((always_inline)) {
@@ -830,20 +875,20 @@ __device__ __forceinline__ void tiled_f64_dma_body(int rt, int cb, const double*
     for (int q = 0; q < 4; ++q) pr = fma(acc[c][q], ring[buf][0][lk + 4 * q][16 * wave + li], pr);
   };
   // stages below the block's diagonal (none in block 0): every tile, no row-dot
-  for (int st = 0; st < jd; ++st) {
-    open(st);
-    body(st % NB, std::integral_constant<int, 0>());
+  for (int q = 0; q < nloop; ++q) {
+    open(q);
+    if (compact || act(q)) body(q % NB, std::integral_constant<int, 0>());
   }
   // The eight diagonal stages.  M is zero below its diagonal: in diagonal stage j tile c still needs
   // products while KS j <= 16 c + 15, i.e. c >= j, and tile j gets its last ones.  Block 0 starts at
   // stage 0 = diagonal stage -jd; its tiles below column 0 are those below -jd, skipped too.
   auto diag = [&](auto jc) __attribute__((always_inline)) {
     constexpr int J = decltype(jc)::value;
-    const int st = jd + J;
-    if (st < 0) return;
-    open(st);
-    body(st % NB, jc);
-    rowdot(st % NB, jc);
+    if (J < j0) return;
+    const int q = nloop + J - j0;
+    open(q);
+    body(q % NB, jc);
+    rowdot(q % NB, jc);
   };
   diag(std::integral_constant<int, 0>());
   diag(std::integral_constant<int, 1>());
@@ -1877,11 +1922,69 @@ __global__ __launch_bounds__(kTWG) void k_contract_f64_rows(const double* __rest
 template <bool DMA>
 __global__ __launch_bounds__(kTWG) void k_contract_f64_tiles(const double* __restrict__ phi, int ld,
                                                              const double* __restrict__ M, int np, int nbf, int nrt,
-                                                             int ncb, double* __restrict__ part) {
+                                                             int ncb, double* __restrict__ part,
+                                                             const uint32_t* __restrict__ amask, int nw) {
   int rt, cb;
   if (!tile_coords(blockIdx.x, nrt, ncb, rt, cb)) return;
-  if (DMA) tiled_f64_dma_body<kF64Ks, 32 / kF64Ks>(rt, cb, phi, ld, M, np, nbf, ncb, part, ld);
+  if (DMA) tiled_f64_dma_body<kF64Ks, 32 / kF64Ks>(rt, cb, phi, ld, M, np, nbf, ncb, part, ld, amask, nw);
   else tiled_f64_body(rt, cb, phi, ld, M, np, part, ld);
+}
+
+// Activity of each row tile's groups of 16 basis functions: bit q of mask[rt nw + q / 32] is set
+// when some point of the tile is within the cutoff (r^2 <= rcut2) of a shell with a function in
+// group q.  A clear bit means the tile's basis values in the group are all exactly zero (the
+// basis-value kernels write 0 beyond a shell's cutoff).  Conservative: the distance is taken to the
+// bounding box of the tile's points, against the cutoff with a relative margin of 1e-6.
+__global__ __launch_bounds__(kTT) void k_tile_mask(MolView m, GeomView g, int64_t v0, int nvox, int nsub, int npts,
+                                                   int nw, uint32_t* __restrict__ mask) {
+  __shared__ double box[kTT / 64][6];
+  __shared__ uint32_t words[64];
+  const int rt = blockIdx.x, t = rt * kTT + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};   // lo xyz, hi xyz
+  if (t < npts) {
+    double x, y, z;
+    chunk_point(g, v0, nvox, nsub, t, x, y, z);
+    b[0] = b[3] = x;
+    b[1] = b[4] = y;
+    b[2] = b[5] = z;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      b[d] = fmin(b[d], __shfl_xor(b[d], off, 64));
+      b[3 + d] = fmax(b[3 + d], __shfl_xor(b[3 + d], off, 64));
+    }
+  if (lane == 0)
+    for (int d = 0; d < 6; ++d) box[wave][d] = b[d];
+  for (int w = threadIdx.x; w < nw; w += kTT) words[w] = 0u;
+  __syncthreads();
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = box[0][d];
+    hi[d] = box[0][3 + d];
+    for (int w = 1; w < kTT / 64; ++w) {
+      lo[d] = fmin(lo[d], box[w][d]);
+      hi[d] = fmax(hi[d], box[w][3 + d]);
+    }
+  }
+  if (lo[0] <= hi[0]) {   // the tile holds at least one point
+    for (int sidx = threadIdx.x; sidx < m.nshell; sidx += kTT) {
+      const DevShell sh = m.shells[sidx];
+      const double4 a = m.atoms[sh.atom];
+      const double c[3] = {a.x, a.y, a.z};
+      double d2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double e = fmax(0.0, fmax(lo[d] - c[d], c[d] - hi[d]));
+        d2 += e * e;
+      }
+      if (d2 <= (double)sh.rcut2 * (1.0 + 1e-6))
+        for (int q = sh.bf >> 4; q <= (sh.bf + 2 * sh.L) >> 4; ++q) atomicOr(&words[q >> 5], 1u << (q & 31));
+    }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < nw; w += kTT) mask[(int64_t)rt * nw + w] = words[w];
 }
 
 // rho_t = scale * sum_cb part[cb][t] (fixed order), optionally / pscale_t
@@ -2187,6 +2290,12 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
   const int ncb = ctx->f64_variant == 3 ? np / kRN : (np + kTT - 1) / kTT;
   double* part = nullptr;
   if (tiled && (rc = qm_ws(ctx, s, WS_PART, sizeof(double) * (size_t)ncb * ldmax, (void**)&part))) return rc;
+  // activity bits of the row tiles' 16-function groups (the LDS-DMA tile kernel skips zero groups)
+  const int nw = (np / 16 + 31) / 32;
+  uint32_t* mask = nullptr;
+  if (ctx->f64_variant == 4 && ctx->tile_skip && nw <= 64 &&
+      (rc = qm_ws(ctx, s, WS_MASK, sizeof(uint32_t) * (size_t)nw * (ldmax / kTT), (void**)&mask)))
+    return rc;
   const GeomView gv = geom_view(g);
   const MolView mv = mol->view();
   const double h3 = g->step * g->step * g->step;
@@ -2206,12 +2315,17 @@ int density_fp64(qm_ctx* ctx, qm_mol* mol, hipStream_t s, const qm_geom* g, int 
       if (ctx->f64_variant == 3)
         hipLaunchKernelGGL(k_contract_f64_rows, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s, (const double*)phi, ld,
                            (const double*)mol->M64, np, nrt, ncb, part);
-      else if (ctx->f64_variant == 4)
+      else if (ctx->f64_variant == 4) {
+        if (mask)
+          hipLaunchKernelGGL(k_tile_mask, dim3(nrt), dim3(kTT), 0, s, mv, gv, v0, nv, nsub, npts, nw, mask);
         hipLaunchKernelGGL(k_contract_f64_tiles<true>, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s,
-                           (const double*)phi, ld, (const double*)mol->M64, np, mol->nbf, nrt, ncb, part);
-      else
+                           (const double*)phi, ld, (const double*)mol->M64, np, mol->nbf, nrt, ncb, part,
+                           (const uint32_t*)mask, nw);
+      } else {
         hipLaunchKernelGGL(k_contract_f64_tiles<false>, dim3(tile_grid(nrt, ncb)), dim3(kTWG), 0, s,
-                           (const double*)phi, ld, (const double*)mol->M64, np, mol->nbf, nrt, ncb, part);
+                           (const double*)phi, ld, (const double*)mol->M64, np, mol->nbf, nrt, ncb, part,
+                           (const uint32_t*)nullptr, 0);
+      }
       QM_CHECK_LAUNCH(ctx);
       if (nsub3 == 1) {
         hipLaunchKernelGGL(k_tiles_finish<TO>, dim3((npts + 255) / 256), dim3(256), 0, s, (const double*)part, ld, ncb,
@@ -2580,6 +2694,11 @@ int qm_tune_set(qm_ctx* ctx, int knob, int value) {
     ctx->jacobi_pairs = value;
     return QM_OK;
   }
+  if (knob == QM_TUNE_TILE_SKIP) {
+    QM_REQUIRE(ctx, value == 0 || value == 1, QM_EINVAL, "tune: tile skip must be 0 or 1");
+    ctx->tile_skip = value;
+    return QM_OK;
+  }
   return qm_fail(ctx, QM_EINVAL, "tune: unknown knob %d", knob);
 }
 
@@ -2601,6 +2720,10 @@ int qm_tune_get(qm_ctx* ctx, int knob, int* value) {
   }
   if (knob == QM_TUNE_JACOBI_PAIRS) {
     *value = ctx->jacobi_pairs;
+    return QM_OK;
+  }
+  if (knob == QM_TUNE_TILE_SKIP) {
+    *value = ctx->tile_skip;
     return QM_OK;
   }
   return qm_fail(ctx, QM_EINVAL, "tune: unknown knob %d", knob);

@@ -72,6 +72,7 @@ enum WsSlot {
   WS_BATCH2,
   WS_VNE,          // diffusion V_ne partials and loop state
   WS_PART,         // per-column-block partial densities of the tiled contraction
+  WS_MASK,         // per-row-tile activity bits of the basis-function groups
   WS_COUNT
 };
 
@@ -115,6 +116,7 @@ struct qm_ctx {
   int f64_variant = 4;                  // FP64 contraction: 0 wave-cyclic, 1 dense per-wave, 2 tiled, 3 rows, 4 tiled + DMA
   int jacobi_zs = 0;                    // z planes per Jacobi sweep workgroup (0 = chosen per grid)
   int jacobi_pairs = 1;                 // FP32 two-sweep passes with two columns per lane (even nx)
+  int tile_skip = 1;                    // contractions skip basis-function groups that are zero on a row tile
   int ncu = 256;                        // compute units of the device
   std::vector<ProfRec> recs;
 };

@@ -82,7 +82,7 @@ SIGNATURES = {
     "qm_profile": [_vp, _i],
     "qm_profile_read": [_vp, _i, _pd, _c.POINTER(_i64)],
 }
-QM_TUNE_SPLIT_VARIANT, QM_TUNE_F64_VARIANT, QM_TUNE_JACOBI_ZS, QM_TUNE_JACOBI_PAIRS = 0, 1, 2, 3
+QM_TUNE_SPLIT_VARIANT, QM_TUNE_F64_VARIANT, QM_TUNE_JACOBI_ZS, QM_TUNE_JACOBI_PAIRS, QM_TUNE_TILE_SKIP = 0, 1, 2, 3, 4
 QM_K_DENSITY_CONTRACT, QM_K_DENSITY_SPLIT, QM_K_PHI, QM_K_PHI_SPLIT, QM_K_FFT_AXIS, QM_K_GTO, QM_K_JACOBI, QM_K_VNE = range(8)
 _RESTYPE = {"qm_last_error": _c.c_char_p}
 

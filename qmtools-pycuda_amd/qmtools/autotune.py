@@ -16,7 +16,8 @@ from . import _device
 from . import _native as nat
 
 _KNOBS = {"split_variant": nat.QM_TUNE_SPLIT_VARIANT, "f64_variant": nat.QM_TUNE_F64_VARIANT,
-          "jacobi_zs": nat.QM_TUNE_JACOBI_ZS, "jacobi_pairs": nat.QM_TUNE_JACOBI_PAIRS}
+          "jacobi_zs": nat.QM_TUNE_JACOBI_ZS, "jacobi_pairs": nat.QM_TUNE_JACOBI_PAIRS,
+          "tile_skip": nat.QM_TUNE_TILE_SKIP}
 _PREC = {"fp64": nat.QM_PREC_FP64, "mixed": nat.QM_PREC_F16X3, "f16": nat.QM_PREC_F16X3,
          "bf16": nat.QM_PREC_BF16X3}
 

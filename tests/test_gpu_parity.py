@@ -570,6 +570,43 @@ def test_translate_reuses_resident_tables(qm, obasis):
     assert dev.stats()["gto_builds"] == 2
 
 
+@pytest.mark.parametrize("subgrid", [1, 2])
+def test_f64_tile_skip_bitwise(qm, subgrid):
+    """The FP64 LDS-DMA contraction skips the products of 16-function groups whose basis values are
+    zero on a whole row tile (beyond every shell's cutoff, by the tile's bounding box): they add exact
+    zeros, so the density equals the one with every product formed, bit for bit.  The workload (60
+    atoms in a 29 bohr box) has such groups: checked on the host for the tiles' bounding boxes."""
+    from qmtools import autotune, synth
+    basis = qm.BasisSet(BASIS)
+    mol = synth.synthetic_molecule(60, 2, basis)
+    grid = synth.centred_grid(mol, 64, step_bohr=0.45, pad_ang=3.0)
+    # host estimate: fraction of (tile, shell) pairs beyond the cutoff for 128-point tiles
+    nx, ny, nz, h, o = grid.geometry()
+    amin = numpy.array([mol.prim_alpha[mol.shell_poff[s]:mol.shell_poff[s] + mol.shell_nprim[s]].min()
+                        for s in range(mol.nshell)])
+    centres = mol.coords[numpy.asarray(mol.shell_atom)]
+    idx = numpy.arange(nx * ny * nz)
+    pts = numpy.stack([o[0] + (idx % nx + 0.5) * h, o[1] + ((idx // nx) % ny + 0.5) * h,
+                       o[2] + (idx // (nx * ny) + 0.5) * h], 1).reshape(-1, 128, 3)
+    lo, hi = pts.min(1), pts.max(1)
+    gap = numpy.maximum(0.0, numpy.maximum(lo[:, None, :] - centres[None], centres[None] - hi[:, None, :]))
+    zero = ((gap ** 2).sum(-1) > 50.0 / amin[None, :]).mean()
+    assert zero > 0.2, zero
+    old = autotune.get("tile_skip"), autotune.get("f64_variant")
+    autotune.set("f64_variant", 4)
+    out = {}
+    try:
+        for skip in (1, 0):
+            autotune.set("tile_skip", skip)
+            _say("f64 tile skip", skip, "subgrid", subgrid, f"zero tile-shell pairs {zero:.2f}")
+            out[skip] = qm.QMTools.Compute_density(grid, mol, subgrid=subgrid).qube.copy()
+    finally:
+        autotune.set("tile_skip", old[0])
+        autotune.set("f64_variant", old[1])
+    assert numpy.abs(out[0]).max() > 0
+    numpy.testing.assert_array_equal(out[1], out[0])
+
+
 def test_gto_table_rebuild_is_deterministic(qm, obasis):
     """The GTO distribution table is built by several host threads, one shell at a time, and
     concatenated in shell order: rebuilding it for the same coordinates (a coordinate update to the

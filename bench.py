@@ -614,7 +614,9 @@ def main():
                      "algorithmic": (f"({'2 nbf^2' if dense else 'nbf (nbf + 1)'} + 2 nbf) flops/point x "
                                      f"{launch_pts:.0f} points/launch"
                                      + (" x 3 split products" if mult > 1 else "")
-                                     + ("" if dense else " (triangular M = D + 2U of symmetric P)")),
+                                     + ("" if dense else " (triangular M = D + 2U of symmetric P)")
+                                     + ("" if mult > 1 else "; counts the whole triangle, of which the kernel "
+                                        "skips the groups of 16 functions that are zero on a row tile")),
                      "dense_equivalent_tflops": dense_equiv,
                      "avg_ms": avg_ms, "launches": k_n},
         "breakdown_ms_per_step": {"contraction": k_ms / args.steps, "phi": phi_ms / args.steps,

@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per contraction launch measured with rocprofv3 FETCH_SIZE (x2 calibration) and
 # WRITE_SIZE (x1) for this workload; source: profiles/mol60_grid256_pmc_traffic.md
 # (precision, nbf_pad, variant): (points per launch, HBM bytes per launch)
-MEASURED_TRAFFIC = {("fp64", 640, "tiles_dma"): (838784, 3904070 * 1024 * 2 + 32765 * 1024),
+MEASURED_TRAFFIC = {("fp64", 640, "tiles_dma"): (838784, 3499848 * 1024 * 2 + 32765 * 1024),
                     ("fp64", 640, "tiles"): (838784, 6138183 * 1024 * 2 + 32765 * 1024),
                     ("fp64", 640, "rows"): (838784, 3870778 * 1024 * 2 + 65536 * 1024),
                     ("fp64", 640, "lds"): (838784, 4444220 * 1024 * 2 + 190037 * 1024),

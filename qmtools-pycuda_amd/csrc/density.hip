@@ -1935,11 +1935,11 @@ __global__ __launch_bounds__(kTWG) void k_contract_f64_tiles(const double* __res
 // group q.  A clear bit means the tile's basis values in the group are all exactly zero (the
 // basis-value kernels write 0 beyond a shell's cutoff).  Conservative: the distance is taken to the
 // bounding box of the tile's points, against the cutoff with a relative margin of 1e-6.
-__global__ __launch_bounds__(kTT) void k_tile_mask(MolView m, GeomView g, int64_t v0, int nvox, int nsub, int npts,
-                                                   int nw, uint32_t* __restrict__ mask) {
+__device__ __forceinline__ void tile_mask_body(const MolView& m, const GeomView& g, int64_t v0, int nvox, int nsub,
+                                               int npts, int rt, int nw, uint32_t* __restrict__ mask) {
   __shared__ double box[kTT / 64][6];
   __shared__ uint32_t words[64];
-  const int rt = blockIdx.x, t = rt * kTT + threadIdx.x;
+  const int t = rt * kTT + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};   // lo xyz, hi xyz
   if (t < npts) {
@@ -1985,6 +1985,11 @@ __global__ __launch_bounds__(kTT) void k_tile_mask(MolView m, GeomView g, int64_
   }
   __syncthreads();
   for (int w = threadIdx.x; w < nw; w += kTT) mask[(int64_t)rt * nw + w] = words[w];
+}
+
+__global__ __launch_bounds__(kTT) void k_tile_mask(MolView m, GeomView g, int64_t v0, int nvox, int nsub, int npts,
+                                                   int nw, uint32_t* __restrict__ mask) {
+  tile_mask_body(m, g, v0, nvox, nsub, npts, blockIdx.x, nw, mask);
 }
 
 // rho_t = scale * sum_cb part[cb][t] (fixed order), optionally / pscale_t
@@ -2814,6 +2819,8 @@ struct BChunk {
   int blk0_phi;         // first block in the round's basis-value (and tiled finish) launch
   int blk0;             // first block in its contraction launch
   const int2* kg;       // fused kernel: the molecule's function-group table
+  uint32_t* mask;       // FP64 LDS-DMA tiles: activity bits of the row tiles' 16-function groups (or null)
+  int nw;               // mask words per row tile
 };
 
 __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ c, int n, int b, bool phi_launch) {
@@ -2919,6 +2926,16 @@ __global__ __launch_bounds__(kTWG) void kb_contract_f64_rows(const BChunk* __res
   tiled_f64_rows_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 
+// activity bits of every chunk of a round, numbered as the basis-value launch (two 128-point tiles
+// per block of that launch); chunks without a mask are left alone
+__global__ __launch_bounds__(kTT) void kb_tile_mask(const BChunk* __restrict__ c, int n) {
+  const int pb = blockIdx.x >> 1;
+  const BChunk& q = c[find_chunk(c, n, pb, true)];
+  const int rt = (pb - q.blk0_phi) * 2 + (blockIdx.x & 1);
+  if (!q.mask || rt >= q.ld / kTT) return;   // uniform per workgroup
+  tile_mask_body(q.mol, q.g, q.v0, q.nvox, 1, q.nvox, rt, q.nw, q.mask);
+}
+
 template <bool DMA>
 __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __restrict__ c, int n) {
   const BChunk& q = c[find_chunk(c, n, blockIdx.x, false)];
@@ -2926,7 +2943,7 @@ __global__ __launch_bounds__(kTWG) void kb_contract_f64_tiles(const BChunk* __re
   if (!tile_coords(blockIdx.x - q.blk0, q.ld / kTT, q.ncb, rt, cb)) return;
   if (DMA)
     tiled_f64_dma_body<kF64Ks, 32 / kF64Ks>(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad,
-                                            q.mol.nbf, q.ncb, q.part, q.ld);
+                                            q.mol.nbf, q.ncb, q.part, q.ld, q.mask, q.nw);
   else tiled_f64_body(rt, cb, (const double*)q.phi, q.ld, (const double*)q.ph, q.nbf_pad, q.part, q.ld);
 }
 
@@ -3124,6 +3141,7 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   std::vector<Round> rounds;
   std::vector<BChunk> cur;
   size_t cur_bytes = 0, cur_ps = 0, cur_part = 0;
+  size_t mask_bytes = 0;   // activity bits of every chunk (FP64 LDS-DMA tiles), kept for the whole call
   auto flush = [&]() {
     if (cur.empty()) return;
     Round r;
@@ -3173,6 +3191,11 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
       c.nbf_pad = np;
       c.nt = tiled ? 0 : pick_nt(np);     // tiled chunks form one launch per round
       c.ncb = (f64 && ctx->f64_variant == 3) ? np / kRN : (np + kTT - 1) / kTT;
+      if (f64 && ctx->f64_variant == 4 && ctx->tile_skip && (np / 16 + 31) / 32 <= 64) {
+        c.nw = (np / 16 + 31) / 32;
+        c.mask = (uint32_t*)(uintptr_t)mask_bytes;   // offset for now, rebased below
+        mask_bytes += sizeof(uint32_t) * (size_t)c.nw * (c.ld / kTT);
+      }
       const size_t bytes = (size_t)np * c.ld * elt;
       if (cur_bytes + bytes > kPhiBudget) flush();
       c.phi = (void*)(uintptr_t)cur_bytes;            // offsets for now, rebased below
@@ -3218,7 +3241,10 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   if ((rc = qm_ws(ctx, s, WS_PHI2, std::max<size_t>(max_ps, 16), &ps))) return rc;
   if (tiled && (rc = qm_ws(ctx, s, WS_PART, max_part, &pt))) return rc;
   if ((rc = qm_ws(ctx, s, WS_BATCH, sizeof(BChunk) * table.size(), &dtab))) return rc;
+  void* mk = nullptr;
+  if (mask_bytes && (rc = qm_ws(ctx, s, WS_MASK, mask_bytes, &mk))) return rc;
   for (BChunk& c : table) {
+    c.mask = c.nw ? (uint32_t*)((char*)mk + (size_t)(uintptr_t)c.mask) : nullptr;
     c.phi = (char*)phi + (size_t)(uintptr_t)c.phi;
     c.pscale = (float*)((char*)ps + (size_t)(uintptr_t)c.pscale);
     c.part = tiled ? (double*)((char*)pt + (size_t)(uintptr_t)c.part) : nullptr;
@@ -3229,9 +3255,11 @@ extern "C" int qm_density_batch(qm_ctx* ctx, void* stream, int nmol, qm_mol* con
   for (const Round& r : rounds) {
     {
       ProfScope pp(ctx, s, f64 ? QM_K_PHI : QM_K_PHI_SPLIT);
-      if (f64)
+      if (f64) {
         hipLaunchKernelGGL(kb_phi_f64, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first, (int)r.count);
-      else if (pre && precision == QM_PREC_F16X3)
+        if (mask_bytes)
+          hipLaunchKernelGGL(kb_tile_mask, dim3(2 * r.phi_blocks), dim3(kTT), 0, s, dt + r.first, (int)r.count);
+      } else if (pre && precision == QM_PREC_F16X3)
         hipLaunchKernelGGL(kb_phi_split<_Float16>, dim3(r.phi_blocks), dim3(kPhiThreads), 0, s, dt + r.first,
                            (int)r.count);
       else if (pre)

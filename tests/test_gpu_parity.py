@@ -572,9 +572,9 @@ def test_translate_reuses_resident_tables(qm, obasis):
 
 @pytest.mark.parametrize("subgrid", [1, 2])
 def test_f64_tile_skip_bitwise(qm, subgrid):
-    """The FP64 LDS-DMA contraction skips the products of 16-function groups whose basis values are
-    zero on a whole row tile (beyond every shell's cutoff, by the tile's bounding box): they add exact
-    zeros, so the density equals the one with every product formed, bit for bit.  The workload (60
+    """The FP64 LDS-DMA contraction (single and batched) skips the products of 16-function groups whose
+    basis values are zero on a whole row tile (beyond every shell's cutoff, by the tile's bounding box):
+    they add exact zeros, so the density equals the one with every product formed, bit for bit.  The workload (60
     atoms in a 29 bohr box) has such groups: checked on the host for the tiles' bounding boxes."""
     from qmtools import autotune, synth
     basis = qm.BasisSet(BASIS)
@@ -600,11 +600,19 @@ def test_f64_tile_skip_bitwise(qm, subgrid):
             autotune.set("tile_skip", skip)
             _say("f64 tile skip", skip, "subgrid", subgrid, f"zero tile-shell pairs {zero:.2f}")
             out[skip] = qm.QMTools.Compute_density(grid, mol, subgrid=subgrid).qube.copy()
+            if subgrid == 1:   # the batched kernel, with a second molecule in the same round
+                from qmtools import batch
+                other = synth.synthetic_molecule(9, 4, basis)
+                qs = batch.density_batch([other, mol], [synth.centred_grid(other, 40, step_bohr=0.5), grid], "fp64")
+                out[(skip, "batch")] = qs[1].cpu().numpy()
     finally:
         autotune.set("tile_skip", old[0])
         autotune.set("f64_variant", old[1])
     assert numpy.abs(out[0]).max() > 0
     numpy.testing.assert_array_equal(out[1], out[0])
+    if subgrid == 1:
+        numpy.testing.assert_array_equal(out[(1, "batch")], out[(0, "batch")])
+        numpy.testing.assert_array_equal(out[(1, "batch")], out[1].ravel())
 
 
 def test_gto_table_rebuild_is_deterministic(qm, obasis):
